@@ -1,0 +1,166 @@
+/*
+ * start_gpu.h — C ABI of libstart_gpu, the MI355X (gfx950) implementation of
+ * Start's per-timestep particle kernel: electromagnetic push in the per-tet
+ * field, tetrahedral relocation, HS/VHS/VSS gas collisions, surface hit and
+ * settle, and barycentric charge scatter to the mesh nodes.
+ *
+ * Drop-in boundary.  Start's step loop (ModelingMainDriver.cpp:132-220) calls
+ *   NodeChargeDensityProcessor::gather  (NodeChargeDensityProcessor.hpp:39-46)
+ *   ParticleDynamicsProcessor::process  (ParticleDynamicsProcessor.hpp:154-170)
+ * once per step.  A C++ shim with those two static signatures forwards to the
+ * entry points below; the library owns every device buffer between steps.
+ *
+ * Conventions: every entry point returns START_OK (0) or a negative status and
+ * never throws; start_last_error() gives the text.  One host thread per
+ * context; calls are stream ordered and only start_read_* and start_step's
+ * statistics synchronise.  All arrays are plain host pointers with the sizes
+ * stated; nothing here takes device memory.
+ */
+#ifndef START_GPU_H
+#define START_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define START_OK 0
+#define START_ERR_ARG -1      /* bad argument / size                     */
+#define START_ERR_HIP -2      /* HIP runtime error (text in last_error)  */
+#define START_ERR_MESH -3     /* degenerate or non-conforming mesh       */
+#define START_ERR_STATE -4    /* call out of order (e.g. no particles)   */
+#define START_ERR_KERNEL -5   /* a kernel reported an internal limit     */
+#define START_ERR_NOMEM -6    /* device allocation failed                */
+
+/* Collision model, chosen by name in the reference (CollisionModelFactory.cpp:23-36). */
+#define START_MODEL_NONE 0
+#define START_MODEL_HS 1
+#define START_MODEL_VHS 2
+#define START_MODEL_VSS 3
+
+/* Arithmetic of the step. */
+#define START_PREC_FP64 0  /* FP64 throughout                                         */
+#define START_PREC_MIXED 1 /* FP32 push/collide/state, FP64 containment near faces and charge */
+
+/* Particle status as read back. */
+#define START_ACTIVE 0
+#define START_SETTLED 1
+
+typedef struct start_ctx start_ctx;
+
+/* Mesh: the tetrahedra of VolumetricMesh (VolumetricMesh.cpp:35-121) and the
+ * surface triangles of SurfaceMesh (GmshUtils.cpp:116-136).  Node ids are
+ * 0-based; triangle id = row index.  Triangles must be faces of the tet mesh
+ * (boundary or internal); boundary faces without a triangle let particles leave. */
+typedef struct start_mesh_desc {
+    int64_t n_nodes;
+    const double* nodes; /* 3*n_nodes, xyz */
+    int64_t n_tets;
+    const int32_t* tets; /* 4*n_tets, any orientation */
+    int64_t n_tris;
+    const int32_t* tris; /* 3*n_tris */
+} start_mesh_desc;
+
+/* Per particle species (ParticlePropertiesManager.hpp:73-217, with overrides). */
+typedef struct start_species_desc {
+    double mass;     /* kg */
+    double charge;   /* C  */
+    double diameter; /* collision diameter r_p + r_t, m (HSModel.cpp:8-12, VHSModel.cpp:14) */
+} start_species_desc;
+
+/* Background gas (Constants.hpp:20-73; ω and α are the float-rounded values). */
+typedef struct start_gas_desc {
+    double mass;
+    double omega;
+    double alpha;
+    double concentration; /* m^-3, see start_gas_concentration */
+    int32_t model;        /* START_MODEL_* */
+    int32_t pad;
+} start_gas_desc;
+
+typedef struct start_params {
+    double dt;              /* "Time Step" (ConfigParser.cpp:37)                          */
+    double bbox_expansion;  /* 1.01 (Constants.hpp:55)                                     */
+    uint64_t seed;          /* variate stream key                                          */
+    uint32_t run;           /* run index folded into the key (batched sweeps)              */
+    int32_t em_push;        /* 1: push with the owning tet's E; 0: the reference's literal
+                               behaviour, where the owner lookup never matches (no push)  */
+    int32_t precision;      /* START_PREC_*                                               */
+    int32_t charge_every_step; /* 1: node charge of the end-of-step positions each step   */
+    int32_t n_species;
+    int32_t steps_per_launch; /* >1 fuses that many steps into one launch (charge then
+                                 accumulates over the fused steps); 0/1 = one step each  */
+    const start_species_desc* species;
+    start_gas_desc gas;
+} start_params;
+
+typedef struct start_stats {
+    uint64_t step;           /* steps taken so far                                   */
+    uint64_t processed;      /* particle-steps processed (not settled, not frozen)   */
+    uint64_t settled;        /* settled particles                                    */
+    uint64_t n_particles;
+    uint64_t deferred;       /* particle-steps resolved by the exact relocation path */
+    uint64_t exhaustive;     /* of those, resolved by the grid search                */
+    int32_t stop;            /* settled >= n_particles (ParticleSettler.cpp:24-25)   */
+    int32_t pad;
+} start_stats;
+
+/* util::calculateConcentration (Utilities.cpp:132-149): n = (P/(R*T))*N_A. */
+double start_gas_concentration(double pressure_pa, double temperature_k);
+
+/* Builds the device mesh (orientation, face adjacency, surface map, barycentric
+ * maps, bucket grids).  Replaces the FEMInitializer/CubicGrid/SurfaceMesh
+ * set-up the step loop relies on (ModelingMainDriver.cpp:111-131). */
+int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_params* params);
+void start_destroy(start_ctx* ctx);
+/* Text of the last failure on ctx (or of the last failed start_create when ctx is NULL). */
+const char* start_last_error(const start_ctx* ctx);
+
+/* Uploads n particles (Particle state, Particle.hpp:25-30) and locates their
+ * owning tets (min-id closed containment).  ids are the caller's particle ids,
+ * kept for start_read_settled; variates are keyed by the load index. */
+int start_load_particles(start_ctx* ctx, int64_t n, const double* x, const double* y,
+                         const double* z, const double* vx, const double* vy, const double* vz,
+                         const int32_t* species, const uint64_t* ids);
+
+/* Per-tet field E (3*n_tets), as ChargeDensityEquationSolver::solve leaves it
+ * (ChargeDensityEquationSolver.cpp:42, VolumetricMesh.cpp:382-411). */
+int start_set_tet_field(start_ctx* ctx, const double* E);
+/* Node potentials (n_nodes) -> per-tet E = -grad(phi) on the device. */
+int start_set_node_potential(start_ctx* ctx, const double* phi);
+
+/* ParticleDynamicsProcessor::process, nsteps times (ParticleDynamicsProcessor.cpp:213-277),
+ * each step followed by the relocation and charge the next gather needs
+ * (NodeChargeDensityProcessor.cpp:169-328).  Asynchronous. */
+int start_step(start_ctx* ctx, int32_t nsteps);
+
+/* NodeChargeDensityProcessor::gather for the current positions: barycentric
+ * node charge Q_n = sum q_p*lambda_n(x_p) into the device node-charge buffer. */
+int start_gather_charge(start_ctx* ctx);
+
+int start_read_node_charge(start_ctx* ctx, double* out /* n_nodes */);
+int start_read_particles(start_ctx* ctx, double* x, double* y, double* z, double* vx, double* vy,
+                         double* vz, int32_t* tet, int32_t* status, int32_t* hit_tri);
+int start_read_tri_counts(start_ctx* ctx, uint64_t* counts /* n_tris */);
+/* Settled ids (ParticlesIDSet, PICTypes.hpp:41) in load order; *n in = capacity, out = count. */
+int start_read_settled(start_ctx* ctx, uint64_t* ids, int64_t* n);
+int start_read_stats(start_ctx* ctx, start_stats* out);
+int start_synchronize(start_ctx* ctx);
+
+/* Device point location of n points (owner rule of the step), for checks. */
+int start_locate(start_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                 int32_t* out_tet);
+/* Device exact orientation signs of n quadruples (12 doubles each), for checks. */
+int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign);
+
+/* Timing hooks: HIP events recorded around the step kernels on the library's
+ * stream; returns the summed kernel milliseconds since the last reset. */
+int start_timing_enable(start_ctx* ctx, int32_t on);
+int start_timing_read(start_ctx* ctx, double* step_kernel_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,797 @@
+/*
+ * start_oracle.c — TEST INFRASTRUCTURE ONLY (see start_oracle.h).
+ *
+ * CPU restatement of the reference per-timestep particle kernel:
+ *   ParticleDynamicsProcessor::_process_ompver__
+ *     (src/ParticleInCellEngine/ParticleDynamicsProcessor/ParticleDynamicsProcessor.cpp:161-200)
+ *   NodeChargeDensityProcessor gather/relocation
+ *     (src/ParticleInCellEngine/NodeChargeDensityProcessor.cpp:51-56, CubicGrid.cpp:59-106)
+ * with the build's documented semantic decisions (owner = minimum-id closed-containing
+ * tetrahedron at step start, barycentric charge scatter, counter-based variates).
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   or_philox4x32_10      Random123 known-answer vectors (Salmon et al., SC'11).
+ *   or_orient3d(_exact)   exact rational determinant (Python fractions) on random,
+ *                         near-degenerate and exactly degenerate points.
+ *   or_seg_tri_intersect  the reference's SegmentTriangleIntersectionTests.cpp:6-44 cases
+ *                         plus brute-force rational tests.
+ *   or_collide            fixed-variate closed forms (HS/VHS/VSS), the reference's
+ *                         ParticleTests.cpp:155-183 probability cases, and statistical
+ *                         invariants (<cos chi> per model, collision frequency).
+ *   or_push / move        uniform-field closed form, q = 0 bitwise no-op,
+ *                         ParticleTests.cpp:99-108.
+ *   or_locate             brute-force minimum-id closed containment over all tets.
+ *   or_first_hit          brute-force over all triangles.
+ *   or_tet_field          linear potential gives the analytic constant field.
+ *   or_scatter_charge     charge conservation, lambda >= 0, brute-force sums.
+ *   or_gas_concentration  closed form n = P*N_A/(R*T) (Utilities.cpp:132-149).
+ *
+ * Compiled with -ffp-contract=off so every + and * is a separately rounded IEEE op.
+ */
+#include "start_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI 3.14159265358979323846 /* START_PI_NUMBER, PreprocessorUtils.hpp:73-77 */
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as   */
+/* easy as 1, 2, 3", SC'11).  Replaces the reference's per-call              */
+/* random_device-seeded mt19937 (HSModel.cpp:18, RealNumberGeneratorHost.cpp:14) */
+/* ------------------------------------------------------------------------- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Two 53-bit uniforms in [0,1) for (particle, step, draw).  Counter =
+ * {pid_lo, pid_hi, step, draw}, key = {seed_lo, seed_hi ^ run}. */
+void or_variates(uint64_t seed, uint32_t run, uint64_t pid, uint32_t step, uint32_t draw, double u[2])
+{
+    uint32_t ctr[4] = {(uint32_t)pid, (uint32_t)(pid >> 32), step, draw};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32) ^ run};
+    uint32_t w[4];
+    or_philox4x32_10(ctr, key, w);
+    const double two_m53 = 1.0 / 9007199254740992.0;
+    uint64_t a = ((uint64_t)w[0] << 21) | (w[1] >> 11);
+    uint64_t b = ((uint64_t)w[2] << 21) | (w[3] >> 11);
+    u[0] = (double)a * two_m53;
+    u[1] = (double)b * two_m53;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Exact geometric predicates.  Sign of det[b-a; c-a; d-a].  A floating-point */
+/* filter (Shewchuk 1997 bound o3derrboundA) answers when it can; otherwise a */
+/* fixed-point big-integer evaluation gives the exact sign.  These reproduce  */
+/* the exact signs CGAL's Epick filtered predicates return                    */
+/* (VolumetricMesh.cpp:23-33, SegmentTriangleIntersection.cpp:7-10).          */
+/* ------------------------------------------------------------------------- */
+#define BI_MAXL 64 /* 32-bit limbs: 2048 bits */
+typedef struct { uint32_t w[BI_MAXL]; } bigi;
+
+static void bi_zero(bigi* r, int nl) { memset(r->w, 0, sizeof(uint32_t) * (size_t)nl); }
+static void bi_neg(bigi* r, int nl)
+{
+    uint64_t carry = 1;
+    for (int i = 0; i < nl; ++i) {
+        uint64_t s = (uint64_t)(~r->w[i]) + carry;
+        r->w[i] = (uint32_t)s;
+        carry = s >> 32;
+    }
+}
+static void bi_add(bigi* r, const bigi* a, const bigi* b, int nl)
+{
+    uint64_t carry = 0;
+    for (int i = 0; i < nl; ++i) {
+        uint64_t s = (uint64_t)a->w[i] + b->w[i] + carry;
+        r->w[i] = (uint32_t)s;
+        carry = s >> 32;
+    }
+}
+static void bi_sub(bigi* r, const bigi* a, const bigi* b, int nl)
+{
+    bigi nb = *b;
+    bi_neg(&nb, nl);
+    bi_add(r, a, &nb, nl);
+}
+/* product modulo 2^(32 nl): equals the two's complement product when it fits */
+static void bi_mul(bigi* r, const bigi* a, const bigi* b, int nl)
+{
+    uint32_t acc[BI_MAXL];
+    memset(acc, 0, sizeof(acc));
+    for (int i = 0; i < nl; ++i) {
+        uint64_t carry = 0;
+        if (a->w[i] == 0) continue;
+        for (int j = 0; i + j < nl; ++j) {
+            uint64_t t = (uint64_t)a->w[i] * b->w[j] + acc[i + j] + carry;
+            acc[i + j] = (uint32_t)t;
+            carry = t >> 32;
+        }
+    }
+    memcpy(r->w, acc, sizeof(uint32_t) * (size_t)nl);
+}
+static int bi_sign(const bigi* a, int nl)
+{
+    if (a->w[nl - 1] & 0x80000000u) return -1;
+    for (int i = 0; i < nl; ++i)
+        if (a->w[i]) return 1;
+    return 0;
+}
+/* value = m * 2^e with |m| < 2^53 (exact decomposition of a double) */
+static void dbl_split(double v, int64_t* m, int* e)
+{
+    if (v == 0.0) { *m = 0; *e = 0; return; }
+    int ex;
+    double f = frexp(v, &ex);
+    *m = (int64_t)ldexp(f, 53);
+    *e = ex - 53;
+}
+static void bi_from(bigi* r, int64_t m, int shift, int nl)
+{
+    bi_zero(r, nl);
+    int neg = m < 0;
+    uint64_t mag = neg ? (uint64_t)(-m) : (uint64_t)m;
+    int limb = shift / 32, bit = shift % 32;
+    /* mag < 2^53 spreads over at most 3 limbs after shifting */
+    unsigned __int128 v = (unsigned __int128)mag << bit;
+    for (int k = 0; k < 3 && limb + k < nl; ++k) r->w[limb + k] = (uint32_t)(v >> (32 * k));
+    if (neg) bi_neg(r, nl);
+}
+
+/* Convert n doubles to big integers on a common binary scale.  Returns limb
+ * count to use (enough for a degree-`deg` polynomial of the values), or -1. */
+static int bi_prepare(const double* v, int n, int deg, bigi* out)
+{
+    int64_t m[12];
+    int e[12], emin = 1 << 30, emax = -(1 << 30);
+    for (int i = 0; i < n; ++i) {
+        dbl_split(v[i], &m[i], &e[i]);
+        if (m[i] != 0) {
+            if (e[i] < emin) emin = e[i];
+            if (e[i] > emax) emax = e[i];
+        }
+    }
+    if (emin > emax) emin = emax = 0;
+    int bits = 54 + (emax - emin) + 2;        /* one value incl. a difference */
+    int need = deg * bits + 8;                 /* products and sums + sign    */
+    int nl = (need + 31) / 32;
+    if (nl > BI_MAXL) return -1;
+    for (int i = 0; i < n; ++i) bi_from(&out[i], m[i], m[i] ? e[i] - emin : 0, nl);
+    return nl;
+}
+
+int or_orient3d_exact(const double* a, const double* b, const double* c, const double* d)
+{
+    double v[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
+    bigi P[12];
+    int nl = bi_prepare(v, 12, 3, P);
+    if (nl < 0) abort(); /* exponent spread beyond 2048-bit budget: never for mesh data */
+    bigi bax, bay, baz, cax, cay, caz, dax, day, daz, t1, t2, m1, s;
+    bi_sub(&bax, &P[3], &P[0], nl); bi_sub(&bay, &P[4], &P[1], nl); bi_sub(&baz, &P[5], &P[2], nl);
+    bi_sub(&cax, &P[6], &P[0], nl); bi_sub(&cay, &P[7], &P[1], nl); bi_sub(&caz, &P[8], &P[2], nl);
+    bi_sub(&dax, &P[9], &P[0], nl); bi_sub(&day, &P[10], &P[1], nl); bi_sub(&daz, &P[11], &P[2], nl);
+    /* det = bax*(cay*daz - caz*day) + bay*(caz*dax - cax*daz) + baz*(cax*day - cay*dax) */
+    bi_mul(&t1, &cay, &daz, nl); bi_mul(&t2, &caz, &day, nl); bi_sub(&m1, &t1, &t2, nl);
+    bi_mul(&s, &bax, &m1, nl);
+    bi_mul(&t1, &caz, &dax, nl); bi_mul(&t2, &cax, &daz, nl); bi_sub(&m1, &t1, &t2, nl);
+    bi_mul(&t1, &bay, &m1, nl); bi_add(&s, &s, &t1, nl);
+    bi_mul(&t1, &cax, &day, nl); bi_mul(&t2, &cay, &dax, nl); bi_sub(&m1, &t1, &t2, nl);
+    bi_mul(&t1, &baz, &m1, nl); bi_add(&s, &s, &t1, nl);
+    return bi_sign(&s, nl);
+}
+
+int or_orient2d_exact(const double* a, const double* b, const double* c)
+{
+    double v[6] = {a[0], a[1], b[0], b[1], c[0], c[1]};
+    bigi P[6];
+    int nl = bi_prepare(v, 6, 2, P);
+    if (nl < 0) abort();
+    bigi bax, bay, cax, cay, t1, t2, s;
+    bi_sub(&bax, &P[2], &P[0], nl); bi_sub(&bay, &P[3], &P[1], nl);
+    bi_sub(&cax, &P[4], &P[0], nl); bi_sub(&cay, &P[5], &P[1], nl);
+    bi_mul(&t1, &bax, &cay, nl); bi_mul(&t2, &bay, &cax, nl); bi_sub(&s, &t1, &t2, nl);
+    return bi_sign(&s, nl);
+}
+
+static double orient3d_fast(const double* a, const double* b, const double* c, const double* d,
+                            double* perm)
+{
+    double bax = b[0] - a[0], bay = b[1] - a[1], baz = b[2] - a[2];
+    double cax = c[0] - a[0], cay = c[1] - a[1], caz = c[2] - a[2];
+    double dax = d[0] - a[0], day = d[1] - a[1], daz = d[2] - a[2];
+    double m1a = cay * daz, m1b = caz * day;
+    double m2a = caz * dax, m2b = cax * daz;
+    double m3a = cax * day, m3b = cay * dax;
+    double det = bax * (m1a - m1b) + bay * (m2a - m2b) + baz * (m3a - m3b);
+    if (perm)
+        *perm = fabs(bax) * (fabs(m1a) + fabs(m1b)) + fabs(bay) * (fabs(m2a) + fabs(m2b)) +
+                fabs(baz) * (fabs(m3a) + fabs(m3b));
+    return det;
+}
+
+int or_orient3d(const double* a, const double* b, const double* c, const double* d)
+{
+    const double eps = 1.1102230246251565e-16; /* 2^-53 */
+    double perm;
+    double det = orient3d_fast(a, b, c, d, &perm);
+    double bound = (7.0 + 56.0 * eps) * eps * perm;
+    if (det > bound) return 1;
+    if (-det > bound) return -1;
+    return or_orient3d_exact(a, b, c, d);
+}
+
+static int orient2d(const double* a, const double* b, const double* c)
+{
+    const double eps = 1.1102230246251565e-16;
+    double l = (b[0] - a[0]) * (c[1] - a[1]);
+    double r = (b[1] - a[1]) * (c[0] - a[0]);
+    double det = l - r;
+    double bound = (3.0 + 16.0 * eps) * eps * (fabs(l) + fabs(r));
+    if (det > bound) return 1;
+    if (-det > bound) return -1;
+    return or_orient2d_exact(a, b, c);
+}
+
+/* closed 2-D segment/segment intersection */
+static int on_seg2(const double* p, const double* q, const double* r)
+{ /* r collinear with pq: inside the closed box? */
+    return fmin(p[0], q[0]) <= r[0] && r[0] <= fmax(p[0], q[0]) && fmin(p[1], q[1]) <= r[1] &&
+           r[1] <= fmax(p[1], q[1]);
+}
+static int seg_seg2(const double* p1, const double* p2, const double* q1, const double* q2)
+{
+    int d1 = orient2d(q1, q2, p1), d2 = orient2d(q1, q2, p2);
+    int d3 = orient2d(p1, p2, q1), d4 = orient2d(p1, p2, q2);
+    if (((d1 > 0 && d2 < 0) || (d1 < 0 && d2 > 0)) && ((d3 > 0 && d4 < 0) || (d3 < 0 && d4 > 0)))
+        return 1;
+    if (d1 == 0 && on_seg2(q1, q2, p1)) return 1;
+    if (d2 == 0 && on_seg2(q1, q2, p2)) return 1;
+    if (d3 == 0 && on_seg2(p1, p2, q1)) return 1;
+    if (d4 == 0 && on_seg2(p1, p2, q2)) return 1;
+    return 0;
+}
+static int pt_in_tri2(const double* p, const double* a, const double* b, const double* c)
+{
+    int o1 = orient2d(a, b, p), o2 = orient2d(b, c, p), o3 = orient2d(c, a, p);
+    return (o1 >= 0 && o2 >= 0 && o3 >= 0) || (o1 <= 0 && o2 <= 0 && o3 <= 0);
+}
+
+/* Closed segment/triangle intersection with exact signs, CGAL::do_intersect
+ * semantics (SegmentTriangleIntersection.cpp:7-10).  On a hit, *t_out is the
+ * parameter along p->q of the intersection point the reference ranks by
+ * distance (ParticleSurfaceCollisionHandler.cpp:104-123); for a coplanar
+ * overlap the reference takes the overlap segment's source
+ * (SegmentTriangleIntersection.cpp:28-30), approximated here by the first
+ * point of the overlap along p->q. */
+int or_seg_tri_intersect(const double* p, const double* q, const double* a, const double* b,
+                         const double* c, double* t_out)
+{
+    int sp = or_orient3d(a, b, c, p), sq = or_orient3d(a, b, c, q);
+    if ((sp > 0 && sq > 0) || (sp < 0 && sq < 0)) return 0;
+    if (sp == 0 && sq == 0) {
+        /* coplanar: project on the plane of the largest normal component */
+        double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+        double wx = c[0] - a[0], wy = c[1] - a[1], wz = c[2] - a[2];
+        double nx = fabs(uy * wz - uz * wy), ny = fabs(uz * wx - ux * wz), nz = fabs(ux * wy - uy * wx);
+        int i0, i1;
+        if (nx >= ny && nx >= nz) { i0 = 1; i1 = 2; }
+        else if (ny >= nz) { i0 = 0; i1 = 2; }
+        else { i0 = 0; i1 = 1; }
+        double P2[2] = {p[i0], p[i1]}, Q2[2] = {q[i0], q[i1]};
+        double A2[2] = {a[i0], a[i1]}, B2[2] = {b[i0], b[i1]}, C2[2] = {c[i0], c[i1]};
+        int hit = pt_in_tri2(P2, A2, B2, C2) || pt_in_tri2(Q2, A2, B2, C2) ||
+                  seg_seg2(P2, Q2, A2, B2) || seg_seg2(P2, Q2, B2, C2) || seg_seg2(P2, Q2, C2, A2);
+        if (!hit) return 0;
+        if (t_out) {
+            if (pt_in_tri2(P2, A2, B2, C2)) { *t_out = 0.0; }
+            else {
+                /* first crossing of the triangle's edge lines along p->q */
+                double best = 1.0;
+                const double* E[3][2] = {{A2, B2}, {B2, C2}, {C2, A2}};
+                for (int k = 0; k < 3; ++k) {
+                    const double* e0 = E[k][0]; const double* e1 = E[k][1];
+                    double ex = e1[0] - e0[0], ey = e1[1] - e0[1];
+                    double dx = Q2[0] - P2[0], dy = Q2[1] - P2[1];
+                    double den = dx * ey - dy * ex;
+                    if (den == 0.0) continue;
+                    double tt = ((e0[0] - P2[0]) * ey - (e0[1] - P2[1]) * ex) / den;
+                    if (tt >= 0.0 && tt < best) best = tt;
+                }
+                *t_out = best;
+            }
+        }
+        return 1;
+    }
+    int o1 = or_orient3d(p, q, a, b), o2 = or_orient3d(p, q, b, c), o3 = or_orient3d(p, q, c, a);
+    int hit = (o1 >= 0 && o2 >= 0 && o3 >= 0) || (o1 <= 0 && o2 <= 0 && o3 <= 0);
+    if (hit && t_out) {
+        if (sp == 0) *t_out = 0.0;
+        else if (sq == 0) *t_out = 1.0;
+        else {
+            double fp = orient3d_fast(a, b, c, p, NULL), fq = orient3d_fast(a, b, c, q, NULL);
+            *t_out = fp / (fp - fq);
+        }
+    }
+    return hit;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Physics constants and derived tables                                      */
+/* ------------------------------------------------------------------------- */
+/* util::calculateConcentration (Utilities.cpp:132-149), R and N_av from
+ * Constants.hpp:10,13: n = (P / (R * T)) * N_av */
+double or_gas_concentration(double pressure, double temperature)
+{
+    const double R = 8.314, N_av = 6.02214076e23;
+    return (pressure / (R * temperature)) * N_av;
+}
+
+void or_species_constants(const or_species_desc* s, const or_gas_desc* g, or_species_k* k)
+{
+    const double KT_reference = 297.0 * 1.380658e-23; /* Constants.hpp:12 */
+    double p_mass = s->mass, t_mass = g->mass, d = s->diameter;
+    k->mass = p_mass;
+    k->charge = s->charge;
+    k->sigma_hs = OR_PI * pow(d, 2);                                  /* HSModel.cpp:12 */
+    k->vhs_c = pow(d, 2) / tgamma(5. / 2. - g->omega);                /* VHSModel.cpp:19 */
+    k->two_kt = 2 * KT_reference;                                     /* VHSModel.cpp:20 */
+    k->m_r = p_mass * t_mass / (p_mass + t_mass);                     /* VHSModel.cpp:15 */
+    k->vhs_exp = g->omega - 1. / 2.;                                  /* VHSModel.cpp:16 */
+    k->mass_cp = p_mass / (t_mass + p_mass);                          /* HSModel.cpp:26 */
+    k->mass_ct = t_mass / (t_mass + p_mass);                          /* HSModel.cpp:27 */
+    k->inv_alpha = g->alpha != 0.0 ? 1. / g->alpha : 0.0;             /* VSSModel.cpp:32 */
+}
+
+/* SurfaceMesh::getGlobalBoundingBox + isParticleWithinModelingSpace
+ * (SurfaceMesh.cpp:92-134): bbox of all non-degenerate triangles, centre and
+ * half extents, half *= expansion; bounds = centre -/+ half. */
+void or_surface_bbox(int64_t nnodes, const double* nodes, int64_t ntris, const int32_t* tris,
+                     double expansion, double lo[3], double hi[3])
+{
+    (void)nnodes;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t t = 0; t < ntris; ++t) {
+        const double* a = nodes + 3 * (int64_t)tris[3 * t];
+        const double* b = nodes + 3 * (int64_t)tris[3 * t + 1];
+        const double* c = nodes + 3 * (int64_t)tris[3 * t + 2];
+        /* Triangle_3::is_degenerate: collinear vertices */
+        double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+        double wx = c[0] - a[0], wy = c[1] - a[1], wz = c[2] - a[2];
+        if (uy * wz - uz * wy == 0.0 && uz * wx - ux * wz == 0.0 && ux * wy - uy * wx == 0.0) continue;
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fmin(mn[k], fmin(a[k], fmin(b[k], c[k])));
+            mx[k] = fmax(mx[k], fmax(a[k], fmax(b[k], c[k])));
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        double centre = (mn[k] + mx[k]) / 2.0;
+        double half = (mx[k] - mn[k]) / 2.0;
+        half *= expansion;
+        lo[k] = centre - half;
+        hi[k] = centre + half;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Single-particle physics                                                   */
+/* ------------------------------------------------------------------------- */
+/* Particle::electroMagneticPush with B = 0 (Particle.cpp:196-234,
+ * ParticlePhysicsUpdater.cpp:4-18).  With B = 0 the rotation terms are exact
+ * zeros, so v = (v + a*dt/2) + a*dt/2 with a = (q*E)/m. */
+void or_push(const double E[3], double q, double m, double dt, double v[3])
+{
+    if (dt == 0.0) return;                                       /* Particle.cpp:199-203 */
+    if (E[0] == 0.0 && E[1] == 0.0 && E[2] == 0.0) return;       /* Particle.cpp:206-207 */
+    for (int k = 0; k < 3; ++k) {
+        double a = (E[k] * q) / m;                               /* Particle.cpp:210 */
+        double h = (a * dt) / 2.;                                /* Particle.cpp:214,230 */
+        v[k] = (v[k] + h) + h;
+    }
+}
+
+/* HSModel::collide (HSModel.cpp:6-42), VHSModel::collide (VHSModel.cpp:6-52),
+ * VSSModel::collide (VSSModel.cpp:6-54).  u1 decides the collision, u2 and u3
+ * feed cos(chi) and phi through uniform_real_distribution's u*(b-a)+a
+ * (RealNumberGeneratorHost.cpp:22).  Returns 1 on a collision. */
+int or_collide(int model, const or_species_k* k, double n, double dt, double v[3], double u1,
+               double u2, double u3)
+{
+    if (model == OR_MODEL_NONE) return 0;
+    double vmod = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double sigma;
+    if (model == OR_MODEL_HS) {
+        sigma = k->sigma_hs;
+    } else {
+        double d_vhs_2 = k->vhs_c * pow(k->two_kt / (k->m_r * vmod * vmod), k->vhs_exp);
+        sigma = OR_PI * d_vhs_2;
+    }
+    double P = sigma * vmod * n * dt;
+    if (!(u1 < P)) return 0;
+    double xi_cos;
+    if (model == OR_MODEL_HS) xi_cos = (u2 * (1.0 - (-1.0))) + (-1.0);
+    else if (model == OR_MODEL_VHS) xi_cos = (u2 * (1.0 - 0.0)) + 0.0;
+    else xi_cos = 2 * pow(u2, k->inv_alpha) - 1.;
+    double xi_sin = sqrt(1 - xi_cos * xi_cos);
+    double phi = (u3 * (2 * OR_PI - 0)) + 0;
+    double x = xi_sin * cos(phi), y = xi_sin * sin(phi), z = xi_cos;
+    double cm[3] = {v[0] * k->mass_cp, v[1] * k->mass_cp, v[2] * k->mass_cp};
+    double pv[3] = {v[0] * k->mass_ct, v[1] * k->mass_ct, v[2] * k->mass_ct};
+    double mp = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
+    double dir[3] = {x * mp, y * mp, z * mp};
+    if (model == OR_MODEL_VSS) {
+        /* GeometryVector::calcBetaGamma + rotation (GeometryVector.hpp:448-474, :66-90) */
+        double beta = acos(pv[2] / mp), gamma = atan2(pv[1], pv[0]);
+        double cb = cos(beta), sb = sin(beta);
+        double tx = cb * dir[0] + sb * dir[2];
+        double tz = -sb * dir[0] + cb * dir[2];
+        dir[0] = tx; dir[2] = tz;
+        double cg = cos(gamma), sg = sin(gamma);
+        tx = cg * dir[0] - sg * dir[1];
+        double ty = sg * dir[0] + cg * dir[1];
+        dir[0] = tx; dir[1] = ty;
+    }
+    v[0] = dir[0] + cm[0];
+    v[1] = dir[1] + cm[1];
+    v[2] = dir[2] + cm[2];
+    return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Mesh: positively oriented tets, uniform bucket grids for tets and surface */
+/* triangles (the role of CubicGrid.cpp:10-106 and the SurfaceMesh AABB tree) */
+/* ------------------------------------------------------------------------- */
+static const int FACE[4][3] = {{1, 3, 2}, {0, 2, 3}, {0, 3, 1}, {0, 1, 2}};
+
+struct or_mesh {
+    int64_t nn, nt, ns;
+    double* nodes;
+    int32_t* tets;
+    int32_t* tris;
+    uint8_t* tri_degenerate;
+    double lo[3], edge;
+    int div[3];
+    int64_t* tcell_start; int32_t* tcell_list;
+    int64_t* scell_start; int32_t* scell_list;
+};
+
+static int cell_coord(const or_mesh* m, int k, double x)
+{
+    double f = (x - m->lo[k]) / m->edge;
+    if (!(f >= 0.0)) return 0;
+    if (f >= (double)m->div[k]) return m->div[k] - 1;
+    int i = (int)f;
+    return i >= m->div[k] ? m->div[k] - 1 : i;
+}
+
+static void build_grid(or_mesh* m, int64_t nitems, int nv, const int32_t* conn, int64_t** start_out,
+                       int32_t** list_out)
+{
+    int64_t ncell = (int64_t)m->div[0] * m->div[1] * m->div[2];
+    int64_t* cnt = (int64_t*)calloc((size_t)ncell + 1, sizeof(int64_t));
+    for (int pass = 0; pass < 2; ++pass) {
+        int32_t* list = pass ? *list_out : NULL;
+        int64_t* fill = NULL;
+        if (pass) {
+            fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncell);
+            memcpy(fill, cnt, sizeof(int64_t) * (size_t)ncell);
+        }
+        for (int64_t t = 0; t < nitems; ++t) {
+            int i0[3], i1[3];
+            for (int k = 0; k < 3; ++k) {
+                double mn = INFINITY, mx = -INFINITY;
+                for (int v = 0; v < nv; ++v) {
+                    double c = m->nodes[3 * (int64_t)conn[nv * t + v] + k];
+                    mn = fmin(mn, c); mx = fmax(mx, c);
+                }
+                i0[k] = cell_coord(m, k, mn); i1[k] = cell_coord(m, k, mx);
+            }
+            for (int a = i0[0]; a <= i1[0]; ++a)
+                for (int b = i0[1]; b <= i1[1]; ++b)
+                    for (int c = i0[2]; c <= i1[2]; ++c) {
+                        int64_t cell = ((int64_t)a * m->div[1] + b) * m->div[2] + c;
+                        if (pass) list[fill[cell]++] = (int32_t)t;
+                        else cnt[cell + 1]++;
+                    }
+        }
+        if (!pass) {
+            for (int64_t c = 0; c < ncell; ++c) cnt[c + 1] += cnt[c];
+            *list_out = (int32_t*)malloc(sizeof(int32_t) * (size_t)(cnt[ncell] + 1));
+        } else {
+            free(fill);
+        }
+    }
+    *start_out = cnt;
+}
+
+or_mesh* or_mesh_build(int64_t nnodes, const double* nodes, int64_t ntets, const int32_t* tets,
+                       int64_t ntris, const int32_t* tris)
+{
+    or_mesh* m = (or_mesh*)calloc(1, sizeof(or_mesh));
+    m->nn = nnodes; m->nt = ntets; m->ns = ntris;
+    m->nodes = (double*)malloc(sizeof(double) * 3 * (size_t)nnodes);
+    memcpy(m->nodes, nodes, sizeof(double) * 3 * (size_t)nnodes);
+    m->tets = (int32_t*)malloc(sizeof(int32_t) * 4 * (size_t)ntets);
+    memcpy(m->tets, tets, sizeof(int32_t) * 4 * (size_t)ntets);
+    m->tris = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(ntris ? ntris : 1));
+    if (ntris) memcpy(m->tris, tris, sizeof(int32_t) * 3 * (size_t)ntris);
+    m->tri_degenerate = (uint8_t*)calloc((size_t)(ntris ? ntris : 1), 1);
+    /* containment is orientation dependent (VolumetricMesh.cpp:23-33): orient
+     * every tet positively so the closed test reads the same for all */
+    for (int64_t t = 0; t < ntets; ++t) {
+        int32_t* v = m->tets + 4 * t;
+        int s = or_orient3d(m->nodes + 3 * (int64_t)v[0], m->nodes + 3 * (int64_t)v[1],
+                            m->nodes + 3 * (int64_t)v[2], m->nodes + 3 * (int64_t)v[3]);
+        if (s < 0) { int32_t tmp = v[2]; v[2] = v[3]; v[3] = tmp; }
+    }
+    for (int64_t s = 0; s < ntris; ++s) {
+        const double* a = m->nodes + 3 * (int64_t)m->tris[3 * s];
+        const double* b = m->nodes + 3 * (int64_t)m->tris[3 * s + 1];
+        const double* c = m->nodes + 3 * (int64_t)m->tris[3 * s + 2];
+        double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+        double wx = c[0] - a[0], wy = c[1] - a[1], wz = c[2] - a[2];
+        m->tri_degenerate[s] =
+            (uy * wz - uz * wy == 0.0 && uz * wx - ux * wz == 0.0 && ux * wy - uy * wx == 0.0);
+    }
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < nnodes; ++i)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fmin(mn[k], m->nodes[3 * i + k]);
+            mx[k] = fmax(mx[k], m->nodes[3 * i + k]);
+        }
+    double vol = (mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]);
+    double edge = cbrt(vol / (double)(ntets > 0 ? ntets : 1) * 6.0);
+    for (int k = 0; k < 3; ++k) {
+        m->lo[k] = mn[k];
+        int d = (int)ceil((mx[k] - mn[k]) / edge);
+        m->div[k] = d < 1 ? 1 : (d > 1024 ? 1024 : d);
+    }
+    m->edge = edge;
+    build_grid(m, ntets, 4, m->tets, &m->tcell_start, &m->tcell_list);
+    build_grid(m, ntris, 3, m->tris, &m->scell_start, &m->scell_list);
+    return m;
+}
+
+void or_mesh_free(or_mesh* m)
+{
+    if (!m) return;
+    free(m->nodes); free(m->tets); free(m->tris); free(m->tri_degenerate);
+    free(m->tcell_start); free(m->tcell_list); free(m->scell_start); free(m->scell_list);
+    free(m);
+}
+
+int64_t or_mesh_ntets(const or_mesh* m) { return m->nt; }
+void or_mesh_tets(const or_mesh* m, int32_t* out) { memcpy(out, m->tets, sizeof(int32_t) * 4 * (size_t)m->nt); }
+
+static const double* NODE(const or_mesh* m, int32_t i) { return m->nodes + 3 * (int64_t)i; }
+
+/* closed containment: all four face orientations >= 0 (VolumetricMesh.cpp:23-33).
+ * returns 0 outside, 1 on the boundary, 2 strictly inside */
+static int tet_contains(const or_mesh* m, int32_t t, const double* p)
+{
+    const int32_t* v = m->tets + 4 * (int64_t)t;
+    int strict = 1;
+    for (int f = 0; f < 4; ++f) {
+        int s = or_orient3d(NODE(m, v[FACE[f][0]]), NODE(m, v[FACE[f][1]]), NODE(m, v[FACE[f][2]]), p);
+        if (s < 0) return 0;
+        if (s == 0) strict = 0;
+    }
+    return strict ? 2 : 1;
+}
+
+/* Owner = minimum-id tet whose closed volume holds p: the grid cell's
+ * candidates are scanned in ascending tet id (CubicGrid.cpp:94-104) and the
+ * tracker lookup returns the first tet in ascending id (CubicGrid.cpp:149-159). */
+int32_t or_locate(const or_mesh* m, const double* p)
+{
+    int a = cell_coord(m, 0, p[0]), b = cell_coord(m, 1, p[1]), c = cell_coord(m, 2, p[2]);
+    int64_t cell = ((int64_t)a * m->div[1] + b) * m->div[2] + c;
+    for (int64_t i = m->tcell_start[cell]; i < m->tcell_start[cell + 1]; ++i) {
+        int32_t t = m->tcell_list[i];
+        if (tet_contains(m, t, p)) return t;
+    }
+    return -1;
+}
+
+void or_locate_many(const or_mesh* m, int64_t n, const double* x, const double* y, const double* z,
+                    int32_t* out)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        double p[3] = {x[i], y[i], z[i]};
+        out[i] = or_locate(m, p);
+    }
+}
+
+/* Closest intersected surface triangle along p->q (ParticleSurfaceCollisionHandler.cpp:21-123):
+ * all non-degenerate triangles the closed segment meets, nearest intersection
+ * to p; equal distances resolve to the smaller triangle id. */
+int32_t or_first_hit(const or_mesh* m, const double* p, const double* q)
+{
+    int i0[3], i1[3];
+    for (int k = 0; k < 3; ++k) {
+        i0[k] = cell_coord(m, k, fmin(p[k], q[k]));
+        i1[k] = cell_coord(m, k, fmax(p[k], q[k]));
+    }
+    int32_t best = -1;
+    double best_t = INFINITY;
+    for (int a = i0[0]; a <= i1[0]; ++a)
+        for (int b = i0[1]; b <= i1[1]; ++b)
+            for (int c = i0[2]; c <= i1[2]; ++c) {
+                int64_t cell = ((int64_t)a * m->div[1] + b) * m->div[2] + c;
+                for (int64_t i = m->scell_start[cell]; i < m->scell_start[cell + 1]; ++i) {
+                    int32_t s = m->scell_list[i];
+                    if (m->tri_degenerate[s]) continue;
+                    const int32_t* v = m->tris + 3 * (int64_t)s;
+                    double t;
+                    if (!or_seg_tri_intersect(p, q, NODE(m, v[0]), NODE(m, v[1]), NODE(m, v[2]), &t)) continue;
+                    if (t < best_t || (t == best_t && s < best)) { best_t = t; best = s; }
+                }
+            }
+    return best;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Field and charge                                                          */
+/* ------------------------------------------------------------------------- */
+/* E = -sum_i phi_i grad(lambda_i); grad(lambda_i) = ((b-a) x (c-a)) / D for
+ * face i = (a,b,c) and D = 6 * volume.  (Physical field; the reference's
+ * VolumetricMesh.cpp:382-411 weighted-gradient variant is not restated.) */
+void or_tet_field(const or_mesh* m, const double* phi, double* E)
+{
+    for (int64_t t = 0; t < m->nt; ++t) {
+        const int32_t* v = m->tets + 4 * t;
+        double D = orient3d_fast(NODE(m, v[0]), NODE(m, v[1]), NODE(m, v[2]), NODE(m, v[3]), NULL);
+        double e[3] = {0, 0, 0};
+        for (int f = 0; f < 4; ++f) {
+            const double* a = NODE(m, v[FACE[f][0]]);
+            const double* b = NODE(m, v[FACE[f][1]]);
+            const double* c = NODE(m, v[FACE[f][2]]);
+            double ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+            double wx = c[0] - a[0], wy = c[1] - a[1], wz = c[2] - a[2];
+            double g[3] = {uy * wz - uz * wy, uz * wx - ux * wz, ux * wy - uy * wx};
+            double ph = phi[v[f]];
+            for (int k = 0; k < 3; ++k) e[k] -= ph * (g[k] / D);
+        }
+        E[3 * t] = e[0]; E[3 * t + 1] = e[1]; E[3 * t + 2] = e[2];
+    }
+}
+
+/* lambda_i(p) = orient3d(face_i, p) / orient3d(tet), as doubles */
+void or_barycentric(const or_mesh* m, int32_t t, const double* p, double lam[4])
+{
+    const int32_t* v = m->tets + 4 * (int64_t)t;
+    double D = orient3d_fast(NODE(m, v[0]), NODE(m, v[1]), NODE(m, v[2]), NODE(m, v[3]), NULL);
+    for (int f = 0; f < 4; ++f)
+        lam[f] = orient3d_fast(NODE(m, v[FACE[f][0]]), NODE(m, v[FACE[f][1]]), NODE(m, v[FACE[f][2]]), p, NULL) / D;
+}
+
+static void atomic_add_double(double* dst, double val)
+{
+#pragma omp atomic
+    *dst += val;
+}
+
+/* Barycentric charge scatter Q_n += q * lambda_n(x) over the owning tet's nodes
+ * (replaces the tet-density average of NodeChargeDensityProcessor.cpp:60-93). */
+static void scatter_one(const or_mesh* m, int32_t t, const double* p, double q, double* node_charge)
+{
+    double lam[4];
+    or_barycentric(m, t, p, lam);
+    const int32_t* v = m->tets + 4 * (int64_t)t;
+    for (int k = 0; k < 4; ++k) atomic_add_double(&node_charge[v[k]], q * lam[k]);
+}
+
+void or_scatter_charge(const or_mesh* m, const or_species_k* sp, int64_t n, const double* x,
+                       const double* y, const double* z, const int32_t* tet, const int32_t* status,
+                       const int32_t* species, double* node_charge)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        if (status[i] == OR_SETTLED || tet[i] < 0) continue;
+        double q = sp[species[i]].charge;
+        if (q == 0.0) continue;
+        double p[3] = {x[i], y[i], z[i]};
+        scatter_one(m, tet[i], p, q, node_charge);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* One step over all particles (ParticleDynamicsProcessor.cpp:161-200), then  */
+/* relocation and charge for the next step's gather                          */
+/* (NodeChargeDensityProcessor.cpp:197-220).                                 */
+/* ------------------------------------------------------------------------- */
+int or_step(const or_mesh* m, const or_params* prm, const or_species_k* sp, const double* E,
+            int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+            int32_t* tet, int32_t* status, int32_t* hit, const int32_t* species,
+            const uint64_t* pid, uint32_t step, uint64_t* tri_count, double* node_charge,
+            int nthreads, uint64_t* processed)
+{
+    uint64_t nproc = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nproc)
+    for (int64_t i = 0; i < n; ++i) {
+        /* 1. settled particles are skipped (ParticleDynamicsProcessor.cpp:167-168) */
+        if (status[i] == OR_SETTLED) continue;
+        double v[3] = {vx[i], vy[i], vz[i]};
+        double p0[3] = {x[i], y[i], z[i]};
+        int32_t t0 = tet[i];
+        int frozen = 0;
+        /* 2. |v| < min_velocity = 10 m/s (ParticleDynamicsProcessor.cpp:171; Constants.hpp:50) */
+        if (sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) < 10.0) frozen = 1;
+        /* 3. outside the 1.01-expanded surface bbox (SurfaceMesh.cpp:104-134) */
+        if (!frozen && !(p0[0] >= prm->bbox_lo[0] && p0[0] <= prm->bbox_hi[0] && p0[1] >= prm->bbox_lo[1] &&
+                         p0[1] <= prm->bbox_hi[1] && p0[2] >= prm->bbox_lo[2] && p0[2] <= prm->bbox_hi[2]))
+            frozen = 1;
+        if (!frozen) {
+            nproc++;
+            const or_species_k* k = &sp[species[i]];
+            /* 4. push with the owning tet's field (ParticleDynamicsProcessor.cpp:179-180) */
+            if (prm->em_push && t0 >= 0) or_push(E + 3 * (int64_t)t0, k->charge, k->mass, prm->dt, v);
+            /* 6. move (Particle.cpp:157-165) */
+            double p1[3] = {p0[0] + v[0] * prm->dt, p0[1] + v[1] * prm->dt, p0[2] + v[2] * prm->dt};
+            /* 6b. degenerate segment: nothing else this step (:188-190) */
+            if (!(p1[0] == p0[0] && p1[1] == p0[1] && p1[2] == p0[2])) {
+                /* 7. gas collision (ParticlePhysicsUpdater.cpp:20-24) */
+                if (prm->model != OR_MODEL_NONE) {
+                    double u[2], w[2];
+                    or_variates(prm->seed, prm->run, pid[i], step, 0, u);
+                    or_variates(prm->seed, prm->run, pid[i], step, 1, w);
+                    or_collide(prm->model, k, prm->concentration, prm->dt, v, u[0], w[0], w[1]);
+                }
+                /* 8. surface hit on the pre-collision segment
+                 *    (ParticleSurfaceCollisionHandler.cpp:6-142; ParticleSettler.cpp:7-26).
+                 *    A segment that stays strictly inside one tet cannot meet a surface
+                 *    triangle (triangles are tet faces), so only the rest is searched. */
+                int32_t t1 = -1;
+                int inside0 = t0 >= 0 ? tet_contains(m, t0, p0) : 0;
+                int inside1 = t0 >= 0 ? tet_contains(m, t0, p1) : 0;
+                int32_t s = -1;
+                if (!(inside0 == 2 && inside1 == 2)) s = or_first_hit(m, p0, p1);
+                if (s >= 0) {
+                    status[i] = OR_SETTLED;
+                    hit[i] = s;
+#pragma omp atomic
+                    tri_count[s] += 1;
+                } else {
+                    t1 = (inside1 == 2) ? t0 : or_locate(m, p1);
+                }
+                x[i] = p1[0]; y[i] = p1[1]; z[i] = p1[2];
+                if (status[i] != OR_SETTLED) tet[i] = t1;
+            }
+            vx[i] = v[0]; vy[i] = v[1]; vz[i] = v[2];
+        }
+        /* charge of the positions the next step's gather sees */
+        if (node_charge && status[i] != OR_SETTLED && tet[i] >= 0) {
+            double q = sp[species[i]].charge;
+            if (q != 0.0) {
+                double p[3] = {x[i], y[i], z[i]};
+                scatter_one(m, tet[i], p, q, node_charge);
+            }
+        }
+    }
+    if (processed) *processed = nproc;
+    return 0;
+}

@@ -1,0 +1,110 @@
+"""ctypes binding of libstart_gpu.so (C ABI in include/start_gpu.h).
+
+The library is built in-tree by start_amd/build.py.  Loading fails loudly if it
+is missing: there is no CPU fallback for the particle step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstart_gpu.so")
+
+START_OK = 0
+MODEL = {"NONE": 0, "HS": 1, "VHS": 2, "VSS": 3}
+PREC_FP64, PREC_MIXED = 0, 1
+ACTIVE, SETTLED = 0, 1
+
+# every entry point declared in include/start_gpu.h
+EXPORTS = [
+    "start_gas_concentration", "start_create", "start_destroy", "start_last_error",
+    "start_load_particles", "start_set_tet_field", "start_set_node_potential", "start_step",
+    "start_gather_charge", "start_read_node_charge", "start_read_particles",
+    "start_read_tri_counts", "start_read_settled", "start_read_stats", "start_synchronize",
+    "start_locate", "start_orient3d_batch", "start_timing_enable", "start_timing_read",
+]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("nodes", C.POINTER(C.c_double)),
+                ("n_tets", C.c_int64), ("tets", C.POINTER(C.c_int32)),
+                ("n_tris", C.c_int64), ("tris", C.POINTER(C.c_int32))]
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [("mass", C.c_double), ("charge", C.c_double), ("diameter", C.c_double)]
+
+
+class GasDesc(C.Structure):
+    _fields_ = [("mass", C.c_double), ("omega", C.c_double), ("alpha", C.c_double),
+                ("concentration", C.c_double), ("model", C.c_int32), ("pad", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("bbox_expansion", C.c_double), ("seed", C.c_uint64),
+                ("run", C.c_uint32), ("em_push", C.c_int32), ("precision", C.c_int32),
+                ("charge_every_step", C.c_int32), ("n_species", C.c_int32),
+                ("steps_per_launch", C.c_int32), ("species", C.POINTER(SpeciesDesc)),
+                ("gas", GasDesc)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("step", C.c_uint64), ("processed", C.c_uint64), ("settled", C.c_uint64),
+                ("n_particles", C.c_uint64), ("deferred", C.c_uint64),
+                ("exhaustive", C.c_uint64), ("stop", C.c_int32), ("pad", C.c_int32)]
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python start_amd/build.py` "
+                           "(there is no CPU fallback for the particle step)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    d, i32, i64, u64, vp = C.c_double, C.c_int32, C.c_int64, C.c_uint64, C.c_void_p
+    L.start_gas_concentration.argtypes = [d, d]
+    L.start_gas_concentration.restype = d
+    L.start_create.argtypes = [P(vp), P(MeshDesc), P(Params)]
+    L.start_destroy.argtypes = [vp]
+    L.start_destroy.restype = None
+    L.start_last_error.argtypes = [vp]
+    L.start_last_error.restype = C.c_char_p
+    L.start_load_particles.argtypes = [vp, i64] + [P(d)] * 6 + [P(i32), P(u64)]
+    L.start_set_tet_field.argtypes = [vp, P(d)]
+    L.start_set_node_potential.argtypes = [vp, P(d)]
+    L.start_step.argtypes = [vp, i32]
+    L.start_gather_charge.argtypes = [vp]
+    L.start_read_node_charge.argtypes = [vp, P(d)]
+    L.start_read_particles.argtypes = [vp] + [P(d)] * 6 + [P(i32)] * 3
+    L.start_read_tri_counts.argtypes = [vp, P(u64)]
+    L.start_read_settled.argtypes = [vp, P(u64), P(i64)]
+    L.start_read_stats.argtypes = [vp, P(Stats)]
+    L.start_synchronize.argtypes = [vp]
+    L.start_locate.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
+    L.start_orient3d_batch.argtypes = [i64, P(d), P(i32)]
+    L.start_timing_enable.argtypes = [vp, i32]
+    L.start_timing_read.argtypes = [vp, P(d), P(u64)]
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if f.restype is C.c_int:  # default
+            f.restype = C.c_int
+    _LIB = L
+    return L
+
+
+class StartError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def check(rc: int, ctx=None):
+    if rc != START_OK:
+        msg = lib().start_last_error(ctx)
+        raise StartError(rc, msg.decode() if msg else "")

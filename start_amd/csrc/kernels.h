@@ -1,0 +1,87 @@
+// Device data layout and kernel argument blocks of the particle step.
+//
+// Particles: structure of arrays in HBM.
+//   x y z vx vy vz   FP64 (FP64 mode) or FP32 (mixed mode)
+//   tet              int32  owning tet at the start of the next step (-1: none)
+//   flags            int32  bit0 settled, bit1 position strictly inside `tet`,
+//                           bits 8..15 species index
+//   hit              int32  surface triangle the particle settled on (-1: none)
+//   pidx             uint32 load index (variate counter)
+// Tets: geo (12 doubles: v0, 3x3 barycentric map), E (3 doubles),
+//   node ids (int4), face codes (int4: >=0 neighbour, -1 open, <=-2 triangle).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mesh_prep.h"
+#include "physics.h"
+
+namespace sg {
+
+constexpr int32_t kSettled = 1;
+constexpr int32_t kStrict = 2;
+constexpr int kSpeciesShift = 8;
+constexpr int kMaxSpecies = 32;
+
+// device counters (unsigned long long)
+enum Counter {
+    kCntProcessed = 0,
+    kCntSettled = 1,
+    kCntDeferred = 2,
+    kCntExhaustive = 3,
+    kCntWork = 4,   // work-list length of the current step
+    kCntError = 5,  // bit 0: expansion capacity, bit 1: work-list overflow
+    kCntTotal = 8
+};
+
+struct WorkItem {
+    double x0, y0, z0;
+    int32_t i;
+    int32_t t0;
+};
+
+struct DevMesh {
+    const double* nodes;   // 3M
+    const double* geo;     // 12T
+    const double* E;       // 3T (FP64)
+    const float* E32;      // 4T (mixed)
+    const float* geo32;    // 12T (mixed)
+    const int4* tnode;     // T
+    const int4* tnbr;      // T
+    const int32_t* tris;   // 3S
+    const uint8_t* tri_degenerate;
+    const int32_t* tgrid_start;
+    const int32_t* tgrid_list;
+    const int32_t* sgrid_start;
+    const int32_t* sgrid_list;
+    GridInfo grid;
+    int64_t n_nodes, n_tets, n_tris;
+    double tau;
+    float tau32;
+};
+
+template <typename R>
+struct DevParticles {
+    R *x, *y, *z, *vx, *vy, *vz;
+    int32_t* tet;
+    int32_t* flags;
+    int32_t* hit;
+    const uint32_t* pidx;
+    int64_t n;
+};
+
+struct StepParams {
+    double dt;
+    double conc;
+    double lo[3], hi[3];
+    uint64_t seed;
+    uint32_t run;
+    uint32_t step;
+    int32_t model;
+    int32_t em_push;
+    int32_t charge;
+    int32_t nspecies;
+};
+
+}  // namespace sg

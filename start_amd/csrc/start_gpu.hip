@@ -1,0 +1,588 @@
+// C ABI of libstart_gpu (declared in include/start_gpu.h): device state of one
+// simulation context and the host side of the step loop.
+#include "../../include/start_gpu.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "mesh_prep.h"
+
+namespace sg {
+void launch_prologue(hipStream_t s, double* node_charge, int64_t n_nodes, unsigned long long* cnt);
+template <typename R>
+void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const StepParams& S, const SpeciesK* sp,
+                 double* node_charge, WorkItem* wl, unsigned long long* cnt, unsigned long long* tri_count,
+                 hipEvent_t ev_mid);
+template <typename R>
+void launch_locate(hipStream_t s, const DevMesh& M, const R* x, const R* y, const R* z, int64_t n, int32_t* tet,
+                   int32_t* flags, unsigned long long* cnt);
+template <typename R>
+void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,
+                   double* node_charge);
+void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32);
+void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
+void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
+}  // namespace sg
+
+using namespace sg;
+
+namespace {
+
+std::string g_create_error;
+
+struct TimedStep {
+    hipEvent_t a0, a1, b1;
+};
+
+}  // namespace
+
+struct start_ctx {
+    std::string err;
+    hipStream_t stream = nullptr;
+    HostMesh hm;
+    start_params prm{};
+    std::vector<SpeciesK> species;
+    // device mesh
+    double* d_nodes = nullptr;
+    double* d_geo = nullptr;
+    double* d_E = nullptr;
+    float* d_E32 = nullptr;
+    int4* d_tnode = nullptr;
+    int4* d_tnbr = nullptr;
+    int32_t* d_tris = nullptr;
+    uint8_t* d_tri_deg = nullptr;
+    int32_t *d_tgs = nullptr, *d_tgl = nullptr, *d_sgs = nullptr, *d_sgl = nullptr;
+    unsigned long long* d_tri_count = nullptr;
+    double* d_node_charge = nullptr;
+    SpeciesK* d_species = nullptr;
+    // particles
+    int64_t n = 0;
+    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_vx = nullptr, *d_vy = nullptr, *d_vz = nullptr;
+    int32_t *d_tet = nullptr, *d_flags = nullptr, *d_hit = nullptr;
+    uint32_t* d_pidx = nullptr;
+    std::vector<uint64_t> ids;
+    WorkItem* d_wl = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    uint64_t step = 0;
+    // timing
+    bool timing = false;
+    std::vector<TimedStep> events;
+    size_t ev_used = 0;
+
+    DevMesh mesh_view() const
+    {
+        DevMesh M{};
+        M.nodes = d_nodes;
+        M.geo = d_geo;
+        M.E = d_E;
+        M.E32 = d_E32;
+        M.geo32 = nullptr;
+        M.tnode = d_tnode;
+        M.tnbr = d_tnbr;
+        M.tris = d_tris;
+        M.tri_degenerate = d_tri_deg;
+        M.tgrid_start = d_tgs;
+        M.tgrid_list = d_tgl;
+        M.sgrid_start = d_sgs;
+        M.sgrid_list = d_sgl;
+        M.grid = hm.grid;
+        M.n_nodes = hm.n_nodes;
+        M.n_tets = hm.n_tets;
+        M.n_tris = hm.n_tris;
+        M.tau = hm.tau;
+        M.tau32 = 0.0f;
+        return M;
+    }
+    DevParticles<double> particles64() const
+    {
+        return DevParticles<double>{d_x, d_y, d_z, d_vx, d_vy, d_vz, d_tet, d_flags, d_hit, d_pidx, n};
+    }
+};
+
+namespace {
+
+int fail(start_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCHECK(ctx, call)                                                                       \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail((ctx), START_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, sizeof(T) * count);
+}
+
+void free_particles(start_ctx* c)
+{
+    void* ptrs[] = {c->d_x, c->d_y, c->d_z, c->d_vx, c->d_vy, c->d_vz, c->d_tet, c->d_flags, c->d_hit, c->d_pidx, c->d_wl};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    c->d_x = c->d_y = c->d_z = c->d_vx = c->d_vy = c->d_vz = nullptr;
+    c->d_tet = c->d_flags = c->d_hit = nullptr;
+    c->d_pidx = nullptr;
+    c->d_wl = nullptr;
+    c->n = 0;
+}
+
+// Species constants with the reference's expressions (HSModel.cpp:8-12,
+// VHSModel.cpp:12-22, VSSModel.cpp:32), evaluated once.
+SpeciesK species_constants(const start_species_desc& s, const start_gas_desc& g)
+{
+    const double KT_reference = 297.0 * 1.380658e-23;  // Constants.hpp:12
+    SpeciesK k{};
+    double p_mass = s.mass, t_mass = g.mass, d = s.diameter;
+    k.mass = p_mass;
+    k.charge = s.charge;
+    k.sigma_hs = kPi * pow(d, 2);
+    k.vhs_c = pow(d, 2) / tgamma(5. / 2. - g.omega);
+    k.two_kt = 2 * KT_reference;
+    k.m_r = p_mass * t_mass / (p_mass + t_mass);
+    k.vhs_exp = g.omega - 1. / 2.;
+    k.mass_cp = p_mass / (t_mass + p_mass);
+    k.mass_ct = t_mass / (t_mass + p_mass);
+    k.inv_alpha = g.alpha != 0.0 ? 1. / g.alpha : 0.0;
+    return k;
+}
+
+int check_kernel_errors(start_ctx* c)
+{
+    unsigned long long h[kCntTotal];
+    HIPCHECK(c, hipMemcpyAsync(h, c->d_cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (h[kCntError] & 1ull) return fail(c, START_ERR_KERNEL, "exact predicate expansion capacity exceeded");
+    return START_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+double start_gas_concentration(double pressure_pa, double temperature_k)
+{
+    const double R = 8.314, N_av = 6.02214076e23;  // Constants.hpp:10,13
+    return (pressure_pa / (R * temperature_k)) * N_av;
+}
+
+const char* start_last_error(const start_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_params* params)
+{
+    if (!out || !mesh || !params) return fail(nullptr, START_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (params->n_species <= 0 || params->n_species > kMaxSpecies || !params->species)
+        return fail(nullptr, START_ERR_ARG, "n_species must be in [1, 32] with a species table");
+    if (params->precision != START_PREC_FP64 && params->precision != START_PREC_MIXED)
+        return fail(nullptr, START_ERR_ARG, "unknown precision");
+    if (params->gas.model < START_MODEL_NONE || params->gas.model > START_MODEL_VSS)
+        return fail(nullptr, START_ERR_ARG, "unknown collision model");
+    if (params->precision == START_PREC_MIXED)
+        return fail(nullptr, START_ERR_ARG, "mixed precision path is not available in this build");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, START_ERR_HIP, "no HIP device available");
+    start_ctx* c = new (std::nothrow) start_ctx();
+    if (!c) return fail(nullptr, START_ERR_NOMEM, "host allocation failed");
+    c->prm = *params;
+    c->prm.species = nullptr;
+    double expansion = params->bbox_expansion > 0.0 ? params->bbox_expansion : 1.01;
+    std::string e = prepare_mesh(c->hm, mesh->n_nodes, mesh->nodes, mesh->n_tets, mesh->tets, mesh->n_tris,
+                                 mesh->tris, expansion);
+    if (!e.empty()) {
+        delete c;
+        return fail(nullptr, START_ERR_MESH, e);
+    }
+    for (int s = 0; s < params->n_species; ++s) c->species.push_back(species_constants(params->species[s], params->gas));
+
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    const HostMesh& hm = c->hm;
+    int64_t T = hm.n_tets, M = hm.n_nodes, S = hm.n_tris;
+    std::vector<int4> tn((size_t)T), tb((size_t)T);
+    for (int64_t t = 0; t < T; ++t) {
+        tn[(size_t)t] = make_int4(hm.tets[4 * t], hm.tets[4 * t + 1], hm.tets[4 * t + 2], hm.tets[4 * t + 3]);
+        tb[(size_t)t] = make_int4(hm.nbr[4 * t], hm.nbr[4 * t + 1], hm.nbr[4 * t + 2], hm.nbr[4 * t + 3]);
+    }
+    hipError_t he = hipSuccess;
+    auto chk = [&](hipError_t x) {
+        if (he == hipSuccess && x != hipSuccess) he = x;
+    };
+    chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    chk(dalloc(&c->d_nodes, 3 * M));
+    chk(dalloc(&c->d_geo, 12 * T));
+    chk(dalloc(&c->d_E, 3 * T));
+    chk(dalloc(&c->d_tnode, T));
+    chk(dalloc(&c->d_tnbr, T));
+    chk(dalloc(&c->d_tris, 3 * S));
+    chk(dalloc(&c->d_tri_deg, S));
+    chk(dalloc(&c->d_tgs, hm.tgrid_start.size()));
+    chk(dalloc(&c->d_tgl, hm.tgrid_list.size()));
+    chk(dalloc(&c->d_sgs, hm.sgrid_start.size()));
+    chk(dalloc(&c->d_sgl, hm.sgrid_list.size()));
+    chk(dalloc(&c->d_tri_count, S));
+    chk(dalloc(&c->d_node_charge, M));
+    chk(dalloc(&c->d_species, c->species.size()));
+    chk(dalloc(&c->d_cnt, kCntTotal));
+    if (he == hipSuccess) {
+        chk(up(c->d_nodes, hm.nodes.data(), sizeof(double) * 3 * M));
+        chk(up(c->d_geo, hm.geo.data(), sizeof(double) * 12 * T));
+        chk(hipMemset(c->d_E, 0, sizeof(double) * 3 * T));
+        chk(up(c->d_tnode, tn.data(), sizeof(int4) * T));
+        chk(up(c->d_tnbr, tb.data(), sizeof(int4) * T));
+        if (S) chk(up(c->d_tris, hm.tris.data(), sizeof(int32_t) * 3 * S));
+        if (S) chk(up(c->d_tri_deg, hm.tri_degenerate.data(), S));
+        chk(up(c->d_tgs, hm.tgrid_start.data(), sizeof(int32_t) * hm.tgrid_start.size()));
+        chk(up(c->d_tgl, hm.tgrid_list.data(), sizeof(int32_t) * hm.tgrid_list.size()));
+        chk(up(c->d_sgs, hm.sgrid_start.data(), sizeof(int32_t) * hm.sgrid_start.size()));
+        chk(up(c->d_sgl, hm.sgrid_list.data(), sizeof(int32_t) * hm.sgrid_list.size()));
+        chk(hipMemset(c->d_tri_count, 0, sizeof(unsigned long long) * std::max<int64_t>(S, 1)));
+        chk(hipMemset(c->d_node_charge, 0, sizeof(double) * M));
+        chk(up(c->d_species, c->species.data(), sizeof(SpeciesK) * c->species.size()));
+        chk(hipMemset(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal));
+    }
+    if (he != hipSuccess) {
+        std::string msg = std::string("device set-up failed: ") + hipGetErrorString(he);
+        start_destroy(c);
+        return fail(nullptr, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP, msg);
+    }
+    *out = c;
+    return START_OK;
+}
+
+void start_destroy(start_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_particles(c);
+    void* ptrs[] = {c->d_nodes, c->d_geo, c->d_E, c->d_E32, c->d_tnode, c->d_tnbr, c->d_tris, c->d_tri_deg,
+                    c->d_tgs, c->d_tgl, c->d_sgs, c->d_sgl, c->d_tri_count, c->d_node_charge, c->d_species, c->d_cnt};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& e : c->events) {
+        (void)hipEventDestroy(e.a0);
+        (void)hipEventDestroy(e.a1);
+        (void)hipEventDestroy(e.b1);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int start_load_particles(start_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                         const double* vx, const double* vy, const double* vz, const int32_t* species,
+                         const uint64_t* ids)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (n < 0 || n > (int64_t)0x7fffffff) return fail(c, START_ERR_ARG, "particle count out of range");
+    if (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !species)) return fail(c, START_ERR_ARG, "null particle array");
+    std::vector<int32_t> flags((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (species[i] < 0 || species[i] >= (int32_t)c->species.size())
+            return fail(c, START_ERR_ARG, "species index out of range at particle " + std::to_string(i));
+        flags[(size_t)i] = species[i] << kSpeciesShift;
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    free_particles(c);
+    hipError_t he = hipSuccess;
+    auto chk = [&](hipError_t e) {
+        if (he == hipSuccess && e != hipSuccess) he = e;
+    };
+    chk(dalloc(&c->d_x, n));
+    chk(dalloc(&c->d_y, n));
+    chk(dalloc(&c->d_z, n));
+    chk(dalloc(&c->d_vx, n));
+    chk(dalloc(&c->d_vy, n));
+    chk(dalloc(&c->d_vz, n));
+    chk(dalloc(&c->d_tet, n));
+    chk(dalloc(&c->d_flags, n));
+    chk(dalloc(&c->d_hit, n));
+    chk(dalloc(&c->d_pidx, n));
+    chk(dalloc(&c->d_wl, n));
+    if (he != hipSuccess) {
+        free_particles(c);
+        return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("particle allocation failed: ") + hipGetErrorString(he));
+    }
+    std::vector<uint32_t> pidx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) pidx[(size_t)i] = (uint32_t)i;
+    std::vector<int32_t> hit((size_t)n, -1);
+    size_t b = sizeof(double) * (size_t)n;
+    if (n > 0) {
+        HIPCHECK(c, hipMemcpy(c->d_x, x, b, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_y, y, b, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_z, z, b, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_vx, vx, b, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_vy, vy, b, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_vz, vz, b, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_flags, flags.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_hit, hit.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->d_pidx, pidx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    }
+    c->n = n;
+    c->ids.assign(ids ? ids : (const uint64_t*)nullptr, ids ? ids + n : (const uint64_t*)nullptr);
+    if (!ids) {
+        c->ids.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) c->ids[(size_t)i] = (uint64_t)i;
+    }
+    HIPCHECK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_tri_count, 0, sizeof(unsigned long long) * std::max<int64_t>(c->hm.n_tris, 1),
+                               c->stream));
+    DevMesh M = c->mesh_view();
+    launch_locate<double>(c->stream, M, c->d_x, c->d_y, c->d_z, n, c->d_tet, c->d_flags, c->d_cnt);
+    HIPCHECK(c, hipGetLastError());
+    c->step = 0;
+    return check_kernel_errors(c);
+}
+
+int start_set_tet_field(start_ctx* c, const double* E)
+{
+    if (!c || !E) return fail(c, START_ERR_ARG, "null argument");
+    HIPCHECK(c, hipMemcpyAsync(c->d_E, E, sizeof(double) * 3 * c->hm.n_tets, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return START_OK;
+}
+
+int start_set_node_potential(start_ctx* c, const double* phi)
+{
+    if (!c || !phi) return fail(c, START_ERR_ARG, "null argument");
+    double* d_phi = nullptr;
+    HIPCHECK(c, dalloc(&d_phi, c->hm.n_nodes));
+    hipError_t e = hipMemcpyAsync(d_phi, phi, sizeof(double) * c->hm.n_nodes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_field(c->stream, c->mesh_view(), d_phi, c->d_E, c->d_E32);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_phi);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("field from potential: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+int start_step(start_ctx* c, int32_t nsteps)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (nsteps < 0) return fail(c, START_ERR_ARG, "negative step count");
+    if (!c->d_x) return fail(c, START_ERR_STATE, "no particles loaded");
+    DevMesh M = c->mesh_view();
+    DevParticles<double> P = c->particles64();
+    StepParams S{};
+    S.dt = c->prm.dt;
+    S.conc = c->prm.gas.concentration;
+    for (int k = 0; k < 3; ++k) {
+        S.lo[k] = c->hm.bbox_lo[k];
+        S.hi[k] = c->hm.bbox_hi[k];
+    }
+    S.seed = c->prm.seed;
+    S.run = c->prm.run;
+    S.model = c->prm.gas.model;
+    S.em_push = c->prm.em_push;
+    S.charge = c->prm.charge_every_step;
+    S.nspecies = (int32_t)c->species.size();
+    double* nc = c->prm.charge_every_step ? c->d_node_charge : nullptr;
+    for (int32_t k = 0; k < nsteps; ++k) {
+        S.step = (uint32_t)c->step;
+        TimedStep* ev = nullptr;
+        if (c->timing) {
+            if (c->ev_used == c->events.size()) {
+                TimedStep t{};
+                HIPCHECK(c, hipEventCreate(&t.a0));
+                HIPCHECK(c, hipEventCreate(&t.a1));
+                HIPCHECK(c, hipEventCreate(&t.b1));
+                c->events.push_back(t);
+            }
+            ev = &c->events[c->ev_used++];
+        }
+        launch_prologue(c->stream, nc, c->hm.n_nodes, c->d_cnt);
+        if (ev) HIPCHECK(c, hipEventRecord(ev->a0, c->stream));
+        launch_step<double>(c->stream, M, P, S, c->d_species, nc, c->d_wl, c->d_cnt, c->d_tri_count,
+                            ev ? ev->a1 : nullptr);
+        if (ev) HIPCHECK(c, hipEventRecord(ev->b1, c->stream));
+        HIPCHECK(c, hipGetLastError());
+        c->step++;
+    }
+    return START_OK;
+}
+
+int start_gather_charge(start_ctx* c)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (!c->d_x) return fail(c, START_ERR_STATE, "no particles loaded");
+    launch_prologue(c->stream, c->d_node_charge, c->hm.n_nodes, c->d_cnt);
+    launch_gather<double>(c->stream, c->mesh_view(), c->particles64(), c->d_species, c->d_node_charge);
+    HIPCHECK(c, hipGetLastError());
+    return START_OK;
+}
+
+int start_read_node_charge(start_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, START_ERR_ARG, "null argument");
+    HIPCHECK(c, hipMemcpyAsync(out, c->d_node_charge, sizeof(double) * c->hm.n_nodes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return check_kernel_errors(c);
+}
+
+int start_read_particles(start_ctx* c, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                         int32_t* tet, int32_t* status, int32_t* hit_tri)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    int64_t n = c->n;
+    size_t b = sizeof(double) * (size_t)n;
+    if (n == 0) return START_OK;
+    hipStream_t s = c->stream;
+    if (x) HIPCHECK(c, hipMemcpyAsync(x, c->d_x, b, hipMemcpyDeviceToHost, s));
+    if (y) HIPCHECK(c, hipMemcpyAsync(y, c->d_y, b, hipMemcpyDeviceToHost, s));
+    if (z) HIPCHECK(c, hipMemcpyAsync(z, c->d_z, b, hipMemcpyDeviceToHost, s));
+    if (vx) HIPCHECK(c, hipMemcpyAsync(vx, c->d_vx, b, hipMemcpyDeviceToHost, s));
+    if (vy) HIPCHECK(c, hipMemcpyAsync(vy, c->d_vy, b, hipMemcpyDeviceToHost, s));
+    if (vz) HIPCHECK(c, hipMemcpyAsync(vz, c->d_vz, b, hipMemcpyDeviceToHost, s));
+    if (tet) HIPCHECK(c, hipMemcpyAsync(tet, c->d_tet, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    if (hit_tri) HIPCHECK(c, hipMemcpyAsync(hit_tri, c->d_hit, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    if (status) {
+        HIPCHECK(c, hipMemcpyAsync(status, c->d_flags, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHECK(c, hipStreamSynchronize(s));
+    if (status)
+        for (int64_t i = 0; i < n; ++i) status[i] = (status[i] & kSettled) ? START_SETTLED : START_ACTIVE;
+    return check_kernel_errors(c);
+}
+
+int start_read_tri_counts(start_ctx* c, uint64_t* counts)
+{
+    if (!c || !counts) return fail(c, START_ERR_ARG, "null argument");
+    if (c->hm.n_tris == 0) return START_OK;
+    HIPCHECK(c, hipMemcpyAsync(counts, c->d_tri_count, sizeof(uint64_t) * c->hm.n_tris, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return START_OK;
+}
+
+int start_read_settled(start_ctx* c, uint64_t* ids, int64_t* n)
+{
+    if (!c || !n) return fail(c, START_ERR_ARG, "null argument");
+    std::vector<int32_t> flags((size_t)c->n);
+    if (c->n) HIPCHECK(c, hipMemcpyAsync(flags.data(), c->d_flags, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    int64_t cap = *n, k = 0;
+    for (int64_t i = 0; i < c->n; ++i)
+        if (flags[(size_t)i] & kSettled) {
+            if (ids && k < cap) ids[k] = c->ids[(size_t)i];
+            ++k;
+        }
+    *n = k;
+    return START_OK;
+}
+
+int start_read_stats(start_ctx* c, start_stats* out)
+{
+    if (!c || !out) return fail(c, START_ERR_ARG, "null argument");
+    unsigned long long h[kCntTotal];
+    HIPCHECK(c, hipMemcpyAsync(h, c->d_cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof(*out));
+    out->step = c->step;
+    out->processed = h[kCntProcessed];
+    out->settled = h[kCntSettled];
+    out->n_particles = (uint64_t)c->n;
+    out->deferred = h[kCntDeferred];
+    out->exhaustive = h[kCntExhaustive];
+    out->stop = (c->n > 0 && h[kCntSettled] >= (unsigned long long)c->n) ? 1 : 0;
+    if (h[kCntError] & 1ull) return fail(c, START_ERR_KERNEL, "exact predicate expansion capacity exceeded");
+    return START_OK;
+}
+
+int start_synchronize(start_ctx* c)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return START_OK;
+}
+
+int start_locate(start_ctx* c, int64_t n, const double* x, const double* y, const double* z, int32_t* out_tet)
+{
+    if (!c || n < 0 || (n > 0 && (!x || !y || !z || !out_tet))) return fail(c, START_ERR_ARG, "bad argument");
+    if (n == 0) return START_OK;
+    double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+    int32_t* dt = nullptr;
+    hipError_t e = dalloc(&dx, n);
+    if (e == hipSuccess) e = dalloc(&dy, n);
+    if (e == hipSuccess) e = dalloc(&dz, n);
+    if (e == hipSuccess) e = dalloc(&dt, n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dz, z, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_locate<double>(c->stream, c->mesh_view(), dx, dy, dz, n, dt, nullptr, c->d_cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out_tet, dt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    (void)hipFree(dz);
+    (void)hipFree(dt);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("locate: ") + hipGetErrorString(e));
+    return check_kernel_errors(c);
+}
+
+int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign)
+{
+    if (n < 0 || (n > 0 && (!pts || !out_sign))) return fail(nullptr, START_ERR_ARG, "bad argument");
+    if (n == 0) return START_OK;
+    double* dp = nullptr;
+    int32_t* ds = nullptr;
+    hipError_t e = dalloc(&dp, 12 * n);
+    if (e == hipSuccess) e = dalloc(&ds, n);
+    if (e == hipSuccess) e = hipMemcpy(dp, pts, sizeof(double) * 12 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_orient_batch(nullptr, dp, n, ds);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out_sign, ds, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(dp);
+    (void)hipFree(ds);
+    if (e != hipSuccess) return fail(nullptr, START_ERR_HIP, std::string("orient3d batch: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+int start_timing_enable(start_ctx* c, int32_t on)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->timing = on != 0;
+    c->ev_used = 0;
+    return START_OK;
+}
+
+int start_timing_read(start_ctx* c, double* step_kernel_ms, uint64_t* launches)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    double main_ms = 0.0;
+    for (size_t k = 0; k < c->ev_used; ++k) {
+        float ms = 0.0f;
+        HIPCHECK(c, hipEventElapsedTime(&ms, c->events[k].a0, c->events[k].a1));
+        main_ms += ms;
+    }
+    if (step_kernel_ms) *step_kernel_ms = main_ms;
+    if (launches) *launches = c->ev_used;
+    c->ev_used = 0;
+    return START_OK;
+}
+
+}  // extern "C"

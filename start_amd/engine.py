@@ -1,0 +1,163 @@
+"""Host-side mirror of the reference step-loop interface over the C ABI.
+
+Start's driver calls, once per step (ModelingMainDriver.cpp:146-184):
+    NodeChargeDensityProcessor::gather(...)   relocation + node charge
+    ChargeDensityEquationSolver::solve(...)   FEM (stays on Trilinos)
+    ParticleDynamicsProcessor::process(...)   push/move/collide/hit/settle
+`ParticleInCellStep` keeps one device context with the particles resident in
+HBM; `gather()` and `process()` keep the reference's names and meaning.
+Errors raise `StartError` (the reference throws from gather,
+NodeChargeDensityProcessor.cpp:103-112, and only logs from process,
+ParticleDynamicsProcessor.cpp:202-209; the C ABI reports both as status codes).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .meshgen import TetMesh
+from .species import BBOX_EXPANSION, Gas, Species
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+class ParticleInCellStep:
+    """One simulation context on the GPU."""
+
+    def __init__(self, mesh: TetMesh, species: list[Species], gas: Gas, dt: float, seed: int,
+                 run: int = 0, em_push: bool = True, precision: str = "fp64",
+                 charge_every_step: bool = True, bbox_expansion: float = BBOX_EXPANSION,
+                 steps_per_launch: int = 1):
+        L = _lib.lib()
+        self.mesh = mesh
+        self._nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
+        self._tets = np.ascontiguousarray(mesh.tets, dtype=np.int32)
+        self._tris = np.ascontiguousarray(mesh.tris, dtype=np.int32)
+        md = _lib.MeshDesc(len(self._nodes), _dp(self._nodes), len(self._tets), _ip(self._tets),
+                           len(self._tris), _ip(self._tris))
+        sp = (_lib.SpeciesDesc * len(species))(*[_lib.SpeciesDesc(s.mass, s.charge, s.diameter)
+                                                 for s in species])
+        self._sp = sp
+        g = _lib.GasDesc(gas.mass, gas.omega, gas.alpha, gas.concentration, _lib.MODEL[gas.model], 0)
+        prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+        prm = _lib.Params(dt, bbox_expansion, seed, run, int(em_push), prec, int(charge_every_step),
+                          len(species), steps_per_launch, sp, g)
+        self.ctx = C.c_void_p()
+        _lib.check(L.start_create(C.byref(self.ctx), C.byref(md), C.byref(prm)), None)
+        self.n = 0
+
+    def close(self):
+        if self.ctx:
+            _lib.lib().start_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        _lib.check(rc, self.ctx)
+
+    # --- inputs -----------------------------------------------------------------
+    def load_particles(self, pos: np.ndarray, vel: np.ndarray, species: np.ndarray, ids=None):
+        pos = np.asarray(pos, dtype=np.float64)
+        vel = np.asarray(vel, dtype=np.float64)
+        cols = [np.ascontiguousarray(a) for a in (pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0],
+                                                  vel[:, 1], vel[:, 2])]
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        idp = None
+        if ids is not None:
+            self._ids = np.ascontiguousarray(ids, dtype=np.uint64)
+            idp = _up(self._ids)
+        self._check(_lib.lib().start_load_particles(self.ctx, len(pos), *[_dp(c) for c in cols],
+                                                    _ip(sp), idp))
+        self.n = len(pos)
+
+    def set_tet_field(self, E: np.ndarray):
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(-1)
+        self._check(_lib.lib().start_set_tet_field(self.ctx, _dp(E)))
+
+    def set_node_potential(self, phi: np.ndarray):
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        self._check(_lib.lib().start_set_node_potential(self.ctx, _dp(phi)))
+
+    # --- the two step-loop calls ------------------------------------------------
+    def process(self, nsteps: int = 1):
+        """ParticleDynamicsProcessor::process, nsteps times (asynchronous)."""
+        self._check(_lib.lib().start_step(self.ctx, int(nsteps)))
+
+    def gather(self):
+        """NodeChargeDensityProcessor::gather for the current positions."""
+        self._check(_lib.lib().start_gather_charge(self.ctx))
+
+    # --- outputs ----------------------------------------------------------------
+    def node_charge(self) -> np.ndarray:
+        out = np.empty(self.mesh.n_nodes, dtype=np.float64)
+        self._check(_lib.lib().start_read_node_charge(self.ctx, _dp(out)))
+        return out
+
+    def particles(self) -> dict:
+        n = self.n
+        f = {k: np.empty(n, dtype=np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        i = {k: np.empty(n, dtype=np.int32) for k in ("tet", "status", "hit")}
+        self._check(_lib.lib().start_read_particles(
+            self.ctx, *[_dp(f[k]) for k in ("x", "y", "z", "vx", "vy", "vz")],
+            *[_ip(i[k]) for k in ("tet", "status", "hit")]))
+        f.update(i)
+        return f
+
+    def tri_counts(self) -> np.ndarray:
+        out = np.zeros(max(self.mesh.n_tris, 1), dtype=np.uint64)
+        self._check(_lib.lib().start_read_tri_counts(self.ctx, _up(out)))
+        return out[: self.mesh.n_tris]
+
+    def settled_ids(self) -> np.ndarray:
+        n = C.c_int64(self.n)
+        out = np.empty(max(self.n, 1), dtype=np.uint64)
+        self._check(_lib.lib().start_read_settled(self.ctx, _up(out), C.byref(n)))
+        return out[: n.value]
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        self._check(_lib.lib().start_read_stats(self.ctx, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _lib.Stats._fields_ if k != "pad"}
+
+    def synchronize(self):
+        self._check(_lib.lib().start_synchronize(self.ctx))
+
+    def locate(self, pts: np.ndarray) -> np.ndarray:
+        pts = np.asarray(pts, dtype=np.float64)
+        x, y, z = (np.ascontiguousarray(pts[:, k]) for k in range(3))
+        out = np.empty(len(pts), dtype=np.int32)
+        self._check(_lib.lib().start_locate(self.ctx, len(pts), _dp(x), _dp(y), _dp(z), _ip(out)))
+        return out
+
+    def timing(self, on: bool):
+        self._check(_lib.lib().start_timing_enable(self.ctx, int(on)))
+
+    def timing_read(self):
+        ms = C.c_double(0)
+        n = C.c_uint64(0)
+        self._check(_lib.lib().start_timing_read(self.ctx, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def orient3d_batch(pts: np.ndarray) -> np.ndarray:
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 12)
+    out = np.empty(len(pts), dtype=np.int32)
+    _lib.check(_lib.lib().start_orient3d_batch(len(pts), _dp(pts), _ip(out)), None)
+    return out

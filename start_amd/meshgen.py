@@ -1,0 +1,95 @@
+"""Structured tetrahedral meshes for the workloads the kernel is measured on.
+
+The reference reads Gmsh meshes (VolumetricMesh.cpp:35-121, GmshUtils.cpp:96-152);
+there is no Gmsh here, so the box workloads are generated directly:
+a structured nx*ny*nz hex grid, every hex split into the 6 Kuhn tetrahedra
+that share its main diagonal (all positively oriented, conforming across hexes),
+and the boundary faces of the tet mesh as the surface triangles.
+
+Node id:   (k*(ny+1) + j)*(nx+1) + i
+Tet id:    6*((k*ny + j)*nx + i) + p,  p indexes the axis permutation
+Triangle:  boundary faces in tet-id order, face f opposite local vertex f.
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass
+
+import numpy as np
+
+# face f of a tet is opposite local vertex f; vertex order makes
+# orient3d(face, opposite vertex) > 0 for a positively oriented tet
+FACES = np.array([[1, 3, 2], [0, 2, 3], [0, 3, 1], [0, 1, 2]], dtype=np.int64)
+
+
+@dataclass
+class TetMesh:
+    nodes: np.ndarray  # (M, 3) float64
+    tets: np.ndarray   # (T, 4) int32
+    tris: np.ndarray   # (S, 3) int32 surface triangles, id = row
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.nodes.shape[0])
+
+    @property
+    def n_tets(self) -> int:
+        return int(self.tets.shape[0])
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.tris.shape[0])
+
+
+def box_kuhn(nx: int, ny: int, nz: int, lx: float, ly: float, lz: float,
+             origin=(0.0, 0.0, 0.0)) -> TetMesh:
+    xs = origin[0] + np.arange(nx + 1, dtype=np.float64) * lx / nx
+    ys = origin[1] + np.arange(ny + 1, dtype=np.float64) * ly / ny
+    zs = origin[2] + np.arange(nz + 1, dtype=np.float64) * lz / nz
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+
+    def nid(i, j, k):
+        return (k * (ny + 1) + j) * (nx + 1) + i
+
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    unit = np.eye(3, dtype=np.int64)
+    tets = np.empty((i.size, 6, 4), dtype=np.int64)
+    for p, perm in enumerate(itertools.permutations(range(3))):
+        c = np.zeros(3, dtype=np.int64)
+        corners = [c.copy()]
+        for ax in perm:
+            c = c + unit[ax]
+            corners.append(c.copy())
+        for v, off in enumerate(corners):
+            tets[:, p, v] = nid(i + off[0], j + off[1], k + off[2])
+    tets = tets.reshape(-1, 4)
+    tets = _orient_positive(nodes, tets)
+    tris = boundary_faces(tets)
+    return TetMesh(nodes=nodes, tets=tets.astype(np.int32), tris=tris.astype(np.int32))
+
+
+def _orient_positive(nodes: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    a, b, c, d = (nodes[tets[:, q]] for q in range(4))
+    det = np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a))
+    tets = tets.copy()
+    neg = det < 0
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    return tets
+
+
+def boundary_faces(tets: np.ndarray) -> np.ndarray:
+    """Faces used by exactly one tet, in (tet id, local face) order."""
+    faces = tets[:, FACES].reshape(-1, 3)
+    key = np.sort(faces, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    once = cnt[inv.ravel()] == 1
+    return faces[once]
+
+
+def linear_potential(mesh: TetMesh, v_bottom: float, v_top: float, axis: int = 2) -> np.ndarray:
+    """Node potential of two Dirichlet plates at the min/max of `axis`."""
+    c = mesh.nodes[:, axis]
+    lo, hi = c.min(), c.max()
+    return v_bottom + (v_top - v_bottom) * (c - lo) / (hi - lo)

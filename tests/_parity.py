@@ -1,0 +1,92 @@
+"""Runs one workload through the CPU oracle and through the GPU C ABI on the
+same inputs and variates, and compares them."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+
+from oracle import pyoracle as O
+from start_amd.engine import ParticleInCellStep
+from start_amd.species import E_CHARGE
+
+# Tolerances (FP64 path):
+#   owning tet, status, hit triangle, triangle counts: bit-exact
+#   x and v: |a - b| <= 1e-12 * |vector| per particle
+#   node charge: |a - b| <= 1e-12 * max(|Q_n|, e)   (atomics add in any order)
+REL_XV = 1e-12
+REL_Q = 1e-12
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+class Pair:
+    def __init__(self, w, n_particles=None, em_push=True, charge=True, E=None):
+        self.w = w
+        n = len(w.pos) if n_particles is None else n_particles
+        self.n = n
+        pos, vel, sp = w.pos[:n], w.vel[:n], w.sp_idx[:n]
+        self.om = O.Mesh(w.mesh.nodes, w.mesh.tets, w.mesh.tris)
+        self.E = self.om.tet_field(w.phi) if E is None else E
+        g = w.gas
+        ks = [O.species_constants(s.mass, s.charge, s.diameter, g.mass, g.omega, g.alpha,
+                                  O.MODEL[g.model]) for s in w.species]
+        tet0 = self.om.locate_many(pos[:, 0], pos[:, 1], pos[:, 2])
+        self.st = O.State.new(pos, vel, sp, np.arange(n, dtype=np.uint64), tet0)
+        self.sim = O.Sim(self.om, ks, O.MODEL[g.model], g.concentration, w.dt, w.seed,
+                         em_push=em_push, E=self.E)
+        self.charge = charge
+        self.qo = np.zeros(w.mesh.n_nodes)
+        self.eng = ParticleInCellStep(w.mesh, w.species, g, w.dt, w.seed, em_push=em_push,
+                                      charge_every_step=charge)
+        self.eng.load_particles(pos, vel, sp)
+        self.eng.set_tet_field(self.E)
+        self.step = 0
+        self.processed_o = 0
+
+    def advance(self, k):
+        for _ in range(k):
+            if self.charge:
+                self.qo[:] = 0.0
+            self.processed_o += self.sim.step(self.st, self.step, self.qo if self.charge else None,
+                                              nthreads=8)
+            self.step += 1
+        self.eng.process(k)
+
+    def compare(self, where=""):
+        g = self.eng.particles()
+        st = self.st
+        bad_tet = np.nonzero(g["tet"] != st.tet)[0]
+        bad_status = np.nonzero(g["status"] != st.status)[0]
+        bad_hit = np.nonzero(g["hit"] != st.hit)[0]
+        msg = f"{where}: tet mismatches {len(bad_tet)} status {len(bad_status)} hit {len(bad_hit)}"
+        assert len(bad_tet) == 0 and len(bad_status) == 0 and len(bad_hit) == 0, \
+            msg + f" first idx {np.r_[bad_tet, bad_status, bad_hit][:5]}"
+        xo = np.stack([st.x, st.y, st.z], 1)
+        xg = np.stack([g["x"], g["y"], g["z"]], 1)
+        vo = np.stack([st.vx, st.vy, st.vz], 1)
+        vg = np.stack([g["vx"], g["vy"], g["vz"]], 1)
+        ex = np.abs(xo - xg).max(axis=1) / np.maximum(np.linalg.norm(xo, axis=1), 1e-300)
+        ev = np.abs(vo - vg).max(axis=1) / np.maximum(np.linalg.norm(vo, axis=1), 1e-300)
+        assert ex.max() <= REL_XV, f"{where}: max rel x error {ex.max():.3e}"
+        assert ev.max() <= REL_XV, f"{where}: max rel v error {ev.max():.3e}"
+        tc = self.eng.tri_counts()
+        assert np.array_equal(tc, self.sim.tri_count[: len(tc)]), f"{where}: triangle counts differ"
+        res = {"max_rel_x": float(ex.max()), "max_rel_v": float(ev.max()),
+               "settled": int((st.status == 1).sum())}
+        if self.charge:
+            qg = self.eng.node_charge()
+            err = np.abs(qg - self.qo) / np.maximum(np.abs(self.qo), E_CHARGE)
+            assert err.max() <= REL_Q, f"{where}: node charge rel error {err.max():.3e}"
+            res["max_rel_q"] = float(err.max())
+        s = self.eng.stats()
+        assert s["processed"] == self.processed_o, f"{where}: processed {s['processed']} vs {self.processed_o}"
+        res.update(processed=s["processed"], deferred=s["deferred"], exhaustive=s["exhaustive"])
+        log(f"{where}: ok {res}")
+        return res
+
+    def close(self):
+        self.eng.close()

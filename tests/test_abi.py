@@ -1,0 +1,51 @@
+"""The C ABI library loads and exports every entry point its header declares
+(no compute: runs without a GPU)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "start_gpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(start_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "start_create" in names and "start_step" in names and len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    from start_amd import _lib
+    L = _lib.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+    assert set(_lib.EXPORTS) == set(_declared())
+
+
+def test_gas_concentration_entry_point():
+    from start_amd import _lib, species
+    assert _lib.lib().start_gas_concentration(1.0, 300.0) == species.concentration(1.0, 300.0)
+
+
+def test_bad_arguments_are_reported_not_thrown():
+    from start_amd import _lib
+    L = _lib.lib()
+    ctx = C.c_void_p()
+    rc = L.start_create(C.byref(ctx), None, None)
+    assert rc == -1
+    assert b"null" in L.start_last_error(None)
+    assert L.start_step(None, 1) == -1
+    assert L.start_read_stats(None, None) == -1
+
+
+def test_oracle_library_loads(oracle_lib):
+    assert oracle_lib.gas_concentration(1.0, 300.0) > 0

@@ -94,6 +94,9 @@ typedef struct start_params {
                                  accumulates over the fused steps); 0/1 = one step each  */
     const start_species_desc* species;
     start_gas_desc gas;
+    int32_t sort_interval;  /* sort particles by tet (and settled last) every this many
+                               steps; 0 = never                                        */
+    int32_t reserved;
 } start_params;
 
 typedef struct start_stats {

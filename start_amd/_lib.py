@@ -46,7 +46,7 @@ class Params(C.Structure):
                 ("run", C.c_uint32), ("em_push", C.c_int32), ("precision", C.c_int32),
                 ("charge_every_step", C.c_int32), ("n_species", C.c_int32),
                 ("steps_per_launch", C.c_int32), ("species", C.POINTER(SpeciesDesc)),
-                ("gas", GasDesc)]
+                ("gas", GasDesc), ("sort_interval", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -30,10 +30,12 @@ enum Counter {
     kCntSettled = 1,
     kCntDeferred = 2,
     kCntExhaustive = 3,
-    kCntWork = 4,   // work-list length of the current step
-    kCntError = 5,  // bit 0: expansion capacity, bit 1: work-list overflow
-    kCntTotal = 8
+    kCntWork = 4,   // work-list length, slot 0 (slot 1 at kCntWork + 2)
+    kCntError = 5,  // bit 0: expansion capacity
+    kCntProcSlot0 = 8,  // processed particle-steps, spread over kProcSlots slots
+    kCntTotal = 8 + 64
 };
+constexpr int kProcSlots = 64;
 
 struct WorkItem {
     double x0, y0, z0;
@@ -82,6 +84,10 @@ struct StepParams {
     int32_t em_push;
     int32_t charge;
     int32_t nspecies;
+    const int64_t* n_active;  // live particles lead the arrays after a sort
+    double* zero_charge;      // node-charge buffer of the next step, cleared here
+    int32_t work_slot;        // work-list counter of this step: cnt[kCntWork + 2*slot]
+    int32_t pad;
 };
 
 }  // namespace sg

@@ -14,7 +14,7 @@
 #include "mesh_prep.h"
 
 namespace sg {
-void launch_prologue(hipStream_t s, double* node_charge, int64_t n_nodes, unsigned long long* cnt);
+void launch_zero_charge(hipStream_t s, double* a, double* b, int64_t n_nodes);
 template <typename R>
 void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const StepParams& S, const SpeciesK* sp,
                  double* node_charge, WorkItem* wl, unsigned long long* cnt, unsigned long long* tri_count,
@@ -28,6 +28,12 @@ void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, co
 void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32);
 void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
+size_t sort_temp_bytes(int64_t n, int nbits);
+template <typename R>
+hipError_t sort_particles(hipStream_t s, const DevParticles<R>& P, R* x, R* y, R* z, R* vx, R* vy, R* vz,
+                          int32_t* tet, int32_t* flags, int32_t* hit, uint32_t* pidx, int32_t* keys,
+                          int32_t* keys_out, int32_t* idx, int32_t* perm, void* temp, size_t temp_bytes,
+                          int32_t n_tets, int nbits, int64_t* n_active);
 }  // namespace sg
 
 using namespace sg;
@@ -59,13 +65,25 @@ struct start_ctx {
     uint8_t* d_tri_deg = nullptr;
     int32_t *d_tgs = nullptr, *d_tgl = nullptr, *d_sgs = nullptr, *d_sgl = nullptr;
     unsigned long long* d_tri_count = nullptr;
-    double* d_node_charge = nullptr;
+    double* d_node_charge = nullptr;  // two buffers of n_nodes: this step's and the next one's
+    int cb = 0;                       // buffer the next step scatters into
+    int latest = 0;                   // buffer holding the most recent node charge
+    int wslot = 0;                    // work-list counter slot of the next step
     SpeciesK* d_species = nullptr;
     // particles
     int64_t n = 0;
     double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_vx = nullptr, *d_vy = nullptr, *d_vz = nullptr;
     int32_t *d_tet = nullptr, *d_flags = nullptr, *d_hit = nullptr;
     uint32_t* d_pidx = nullptr;
+    // second set for the sort, and sort scratch
+    double *d_x2 = nullptr, *d_y2 = nullptr, *d_z2 = nullptr, *d_vx2 = nullptr, *d_vy2 = nullptr, *d_vz2 = nullptr;
+    int32_t *d_tet2 = nullptr, *d_flags2 = nullptr, *d_hit2 = nullptr;
+    uint32_t* d_pidx2 = nullptr;
+    int32_t *d_keys = nullptr, *d_keys_out = nullptr, *d_idx = nullptr, *d_perm = nullptr;
+    void* d_sort_temp = nullptr;
+    size_t sort_temp = 0;
+    int sort_bits = 1;
+    int64_t* d_nactive = nullptr;
     std::vector<uint64_t> ids;
     WorkItem* d_wl = nullptr;
     unsigned long long* d_cnt = nullptr;
@@ -131,14 +149,43 @@ hipError_t dalloc(T** p, size_t count)
 
 void free_particles(start_ctx* c)
 {
-    void* ptrs[] = {c->d_x, c->d_y, c->d_z, c->d_vx, c->d_vy, c->d_vz, c->d_tet, c->d_flags, c->d_hit, c->d_pidx, c->d_wl};
+    void* ptrs[] = {c->d_x,     c->d_y,     c->d_z,     c->d_vx,     c->d_vy,       c->d_vz,  c->d_tet,
+                    c->d_flags, c->d_hit,   c->d_pidx,  c->d_wl,     c->d_x2,       c->d_y2,  c->d_z2,
+                    c->d_vx2,   c->d_vy2,   c->d_vz2,   c->d_tet2,   c->d_flags2,   c->d_hit2, c->d_pidx2,
+                    c->d_keys,  c->d_keys_out, c->d_idx, c->d_perm,  c->d_sort_temp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c->d_x = c->d_y = c->d_z = c->d_vx = c->d_vy = c->d_vz = nullptr;
-    c->d_tet = c->d_flags = c->d_hit = nullptr;
-    c->d_pidx = nullptr;
+    c->d_x2 = c->d_y2 = c->d_z2 = c->d_vx2 = c->d_vy2 = c->d_vz2 = nullptr;
+    c->d_tet = c->d_flags = c->d_hit = c->d_tet2 = c->d_flags2 = c->d_hit2 = nullptr;
+    c->d_pidx = c->d_pidx2 = nullptr;
+    c->d_keys = c->d_keys_out = c->d_idx = c->d_perm = nullptr;
+    c->d_sort_temp = nullptr;
+    c->sort_temp = 0;
     c->d_wl = nullptr;
     c->n = 0;
+}
+
+// radix sort by owning tet, settled particles last, then swap the buffer sets
+int sort_now(start_ctx* c)
+{
+    if (c->n == 0) return START_OK;
+    hipError_t e = sort_particles<double>(c->stream, c->particles64(), c->d_x2, c->d_y2, c->d_z2, c->d_vx2, c->d_vy2,
+                                          c->d_vz2, c->d_tet2, c->d_flags2, c->d_hit2, c->d_pidx2, c->d_keys,
+                                          c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp, c->sort_temp,
+                                          (int32_t)c->hm.n_tets, c->sort_bits, c->d_nactive);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("particle sort: ") + hipGetErrorString(e));
+    std::swap(c->d_x, c->d_x2);
+    std::swap(c->d_y, c->d_y2);
+    std::swap(c->d_z, c->d_z2);
+    std::swap(c->d_vx, c->d_vx2);
+    std::swap(c->d_vy, c->d_vy2);
+    std::swap(c->d_vz, c->d_vz2);
+    std::swap(c->d_tet, c->d_tet2);
+    std::swap(c->d_flags, c->d_flags2);
+    std::swap(c->d_hit, c->d_hit2);
+    std::swap(c->d_pidx, c->d_pidx2);
+    return START_OK;
 }
 
 // Species constants with the reference's expressions (HSModel.cpp:8-12,
@@ -237,9 +284,10 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
     chk(dalloc(&c->d_sgs, hm.sgrid_start.size()));
     chk(dalloc(&c->d_sgl, hm.sgrid_list.size()));
     chk(dalloc(&c->d_tri_count, S));
-    chk(dalloc(&c->d_node_charge, M));
+    chk(dalloc(&c->d_node_charge, 2 * M));
     chk(dalloc(&c->d_species, c->species.size()));
     chk(dalloc(&c->d_cnt, kCntTotal));
+    chk(dalloc(&c->d_nactive, 1));
     if (he == hipSuccess) {
         chk(up(c->d_nodes, hm.nodes.data(), sizeof(double) * 3 * M));
         chk(up(c->d_geo, hm.geo.data(), sizeof(double) * 12 * T));
@@ -253,7 +301,7 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         chk(up(c->d_sgs, hm.sgrid_start.data(), sizeof(int32_t) * hm.sgrid_start.size()));
         chk(up(c->d_sgl, hm.sgrid_list.data(), sizeof(int32_t) * hm.sgrid_list.size()));
         chk(hipMemset(c->d_tri_count, 0, sizeof(unsigned long long) * std::max<int64_t>(S, 1)));
-        chk(hipMemset(c->d_node_charge, 0, sizeof(double) * M));
+        chk(hipMemset(c->d_node_charge, 0, sizeof(double) * 2 * M));
         chk(up(c->d_species, c->species.data(), sizeof(SpeciesK) * c->species.size()));
         chk(hipMemset(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal));
     }
@@ -272,7 +320,8 @@ void start_destroy(start_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_particles(c);
     void* ptrs[] = {c->d_nodes, c->d_geo, c->d_E, c->d_E32, c->d_tnode, c->d_tnbr, c->d_tris, c->d_tri_deg,
-                    c->d_tgs, c->d_tgl, c->d_sgs, c->d_sgl, c->d_tri_count, c->d_node_charge, c->d_species, c->d_cnt};
+                    c->d_tgs, c->d_tgl, c->d_sgs, c->d_sgl, c->d_tri_count, c->d_node_charge, c->d_species, c->d_cnt,
+                    c->d_nactive};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->events) {
@@ -314,6 +363,27 @@ int start_load_particles(start_ctx* c, int64_t n, const double* x, const double*
     chk(dalloc(&c->d_hit, n));
     chk(dalloc(&c->d_pidx, n));
     chk(dalloc(&c->d_wl, n));
+    if (c->prm.sort_interval > 0) {
+        chk(dalloc(&c->d_x2, n));
+        chk(dalloc(&c->d_y2, n));
+        chk(dalloc(&c->d_z2, n));
+        chk(dalloc(&c->d_vx2, n));
+        chk(dalloc(&c->d_vy2, n));
+        chk(dalloc(&c->d_vz2, n));
+        chk(dalloc(&c->d_tet2, n));
+        chk(dalloc(&c->d_flags2, n));
+        chk(dalloc(&c->d_hit2, n));
+        chk(dalloc(&c->d_pidx2, n));
+        chk(dalloc(&c->d_keys, n));
+        chk(dalloc(&c->d_keys_out, n));
+        chk(dalloc(&c->d_idx, n));
+        chk(dalloc(&c->d_perm, n));
+        int bits = 1;
+        while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
+        c->sort_bits = bits;
+        c->sort_temp = sort_temp_bytes(n, bits);
+        chk(hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16)));
+    }
     if (he != hipSuccess) {
         free_particles(c);
         return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
@@ -343,9 +413,19 @@ int start_load_particles(start_ctx* c, int64_t n, const double* x, const double*
     HIPCHECK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_tri_count, 0, sizeof(unsigned long long) * std::max<int64_t>(c->hm.n_tris, 1),
                                c->stream));
+    int64_t nact = n;
+    HIPCHECK(c, hipMemcpyAsync(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_node_charge, 0, sizeof(double) * 2 * c->hm.n_nodes, c->stream));
+    c->cb = 0;
+    c->latest = 0;
+    c->wslot = 0;
     DevMesh M = c->mesh_view();
     launch_locate<double>(c->stream, M, c->d_x, c->d_y, c->d_z, n, c->d_tet, c->d_flags, c->d_cnt);
     HIPCHECK(c, hipGetLastError());
+    if (c->prm.sort_interval > 0) {
+        int rc = sort_now(c);
+        if (rc != START_OK) return rc;
+    }
     c->step = 0;
     return check_kernel_errors(c);
 }
@@ -394,8 +474,17 @@ int start_step(start_ctx* c, int32_t nsteps)
     S.em_push = c->prm.em_push;
     S.charge = c->prm.charge_every_step;
     S.nspecies = (int32_t)c->species.size();
-    double* nc = c->prm.charge_every_step ? c->d_node_charge : nullptr;
+    S.n_active = c->d_nactive;
+    const int64_t Mn = c->hm.n_nodes;
     for (int32_t k = 0; k < nsteps; ++k) {
+        double* nc = c->prm.charge_every_step ? c->d_node_charge + (size_t)c->cb * Mn : nullptr;
+        S.zero_charge = c->prm.charge_every_step ? c->d_node_charge + (size_t)(1 - c->cb) * Mn : nullptr;
+        S.work_slot = c->wslot;
+        if (c->prm.sort_interval > 0 && c->step > 0 && c->step % (uint64_t)c->prm.sort_interval == 0) {
+            int rc = sort_now(c);
+            if (rc != START_OK) return rc;
+            P = c->particles64();
+        }
         S.step = (uint32_t)c->step;
         TimedStep* ev = nullptr;
         if (c->timing) {
@@ -408,13 +497,17 @@ int start_step(start_ctx* c, int32_t nsteps)
             }
             ev = &c->events[c->ev_used++];
         }
-        launch_prologue(c->stream, nc, c->hm.n_nodes, c->d_cnt);
         if (ev) HIPCHECK(c, hipEventRecord(ev->a0, c->stream));
         launch_step<double>(c->stream, M, P, S, c->d_species, nc, c->d_wl, c->d_cnt, c->d_tri_count,
                             ev ? ev->a1 : nullptr);
         if (ev) HIPCHECK(c, hipEventRecord(ev->b1, c->stream));
         HIPCHECK(c, hipGetLastError());
         c->step++;
+        c->wslot ^= 1;
+        if (nc) {
+            c->latest = c->cb;
+            c->cb ^= 1;
+        }
     }
     return START_OK;
 }
@@ -423,16 +516,21 @@ int start_gather_charge(start_ctx* c)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
     if (!c->d_x) return fail(c, START_ERR_STATE, "no particles loaded");
-    launch_prologue(c->stream, c->d_node_charge, c->hm.n_nodes, c->d_cnt);
-    launch_gather<double>(c->stream, c->mesh_view(), c->particles64(), c->d_species, c->d_node_charge);
+    const int64_t Mn = c->hm.n_nodes;
+    double* cur = c->d_node_charge + (size_t)c->cb * Mn;
+    launch_zero_charge(c->stream, cur, c->d_node_charge + (size_t)(1 - c->cb) * Mn, Mn);
+    launch_gather<double>(c->stream, c->mesh_view(), c->particles64(), c->d_species, cur);
     HIPCHECK(c, hipGetLastError());
+    c->latest = c->cb;
+    c->cb ^= 1;
     return START_OK;
 }
 
 int start_read_node_charge(start_ctx* c, double* out)
 {
     if (!c || !out) return fail(c, START_ERR_ARG, "null argument");
-    HIPCHECK(c, hipMemcpyAsync(out, c->d_node_charge, sizeof(double) * c->hm.n_nodes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(out, c->d_node_charge + (size_t)c->latest * c->hm.n_nodes,
+                               sizeof(double) * c->hm.n_nodes, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return check_kernel_errors(c);
 }
@@ -442,23 +540,34 @@ int start_read_particles(start_ctx* c, double* x, double* y, double* z, double* 
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
     int64_t n = c->n;
-    size_t b = sizeof(double) * (size_t)n;
     if (n == 0) return START_OK;
     hipStream_t s = c->stream;
-    if (x) HIPCHECK(c, hipMemcpyAsync(x, c->d_x, b, hipMemcpyDeviceToHost, s));
-    if (y) HIPCHECK(c, hipMemcpyAsync(y, c->d_y, b, hipMemcpyDeviceToHost, s));
-    if (z) HIPCHECK(c, hipMemcpyAsync(z, c->d_z, b, hipMemcpyDeviceToHost, s));
-    if (vx) HIPCHECK(c, hipMemcpyAsync(vx, c->d_vx, b, hipMemcpyDeviceToHost, s));
-    if (vy) HIPCHECK(c, hipMemcpyAsync(vy, c->d_vy, b, hipMemcpyDeviceToHost, s));
-    if (vz) HIPCHECK(c, hipMemcpyAsync(vz, c->d_vz, b, hipMemcpyDeviceToHost, s));
-    if (tet) HIPCHECK(c, hipMemcpyAsync(tet, c->d_tet, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
-    if (hit_tri) HIPCHECK(c, hipMemcpyAsync(hit_tri, c->d_hit, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
-    if (status) {
-        HIPCHECK(c, hipMemcpyAsync(status, c->d_flags, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    // device order differs from load order after a sort: bring back by load index
+    std::vector<uint32_t> pidx((size_t)n);
+    HIPCHECK(c, hipMemcpyAsync(pidx.data(), c->d_pidx, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    std::vector<double> tmp((size_t)n);
+    std::vector<int32_t> itmp((size_t)n);
+    double* dsrc[6] = {c->d_x, c->d_y, c->d_z, c->d_vx, c->d_vy, c->d_vz};
+    double* ddst[6] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < 6; ++k) {
+        if (!ddst[k]) continue;
+        HIPCHECK(c, hipMemcpyAsync(tmp.data(), dsrc[k], sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(c, hipStreamSynchronize(s));
+        for (int64_t j = 0; j < n; ++j) ddst[k][pidx[(size_t)j]] = tmp[(size_t)j];
+    }
+    int32_t* isrc[3] = {c->d_tet, c->d_flags, c->d_hit};
+    int32_t* idst[3] = {tet, status, hit_tri};
+    for (int k = 0; k < 3; ++k) {
+        if (!idst[k]) continue;
+        HIPCHECK(c, hipMemcpyAsync(itmp.data(), isrc[k], sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(c, hipStreamSynchronize(s));
+        for (int64_t j = 0; j < n; ++j) {
+            int32_t v = itmp[(size_t)j];
+            if (k == 1) v = (v & kSettled) ? START_SETTLED : START_ACTIVE;
+            idst[k][pidx[(size_t)j]] = v;
+        }
     }
     HIPCHECK(c, hipStreamSynchronize(s));
-    if (status)
-        for (int64_t i = 0; i < n; ++i) status[i] = (status[i] & kSettled) ? START_SETTLED : START_ACTIVE;
     return check_kernel_errors(c);
 }
 
@@ -476,11 +585,18 @@ int start_read_settled(start_ctx* c, uint64_t* ids, int64_t* n)
 {
     if (!c || !n) return fail(c, START_ERR_ARG, "null argument");
     std::vector<int32_t> flags((size_t)c->n);
-    if (c->n) HIPCHECK(c, hipMemcpyAsync(flags.data(), c->d_flags, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> pidx((size_t)c->n);
+    if (c->n) {
+        HIPCHECK(c, hipMemcpyAsync(flags.data(), c->d_flags, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(pidx.data(), c->d_pidx, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<uint8_t> settled((size_t)c->n, 0);
+    for (int64_t j = 0; j < c->n; ++j)
+        if (flags[(size_t)j] & kSettled) settled[pidx[(size_t)j]] = 1;
     int64_t cap = *n, k = 0;
     for (int64_t i = 0; i < c->n; ++i)
-        if (flags[(size_t)i] & kSettled) {
+        if (settled[(size_t)i]) {
             if (ids && k < cap) ids[k] = c->ids[(size_t)i];
             ++k;
         }
@@ -496,7 +612,8 @@ int start_read_stats(start_ctx* c, start_stats* out)
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     memset(out, 0, sizeof(*out));
     out->step = c->step;
-    out->processed = h[kCntProcessed];
+    out->processed = 0;
+    for (int k = 0; k < kProcSlots; ++k) out->processed += h[kCntProcSlot0 + k];
     out->settled = h[kCntSettled];
     out->n_particles = (uint64_t)c->n;
     out->deferred = h[kCntDeferred];

@@ -39,7 +39,7 @@ class ParticleInCellStep:
     def __init__(self, mesh: TetMesh, species: list[Species], gas: Gas, dt: float, seed: int,
                  run: int = 0, em_push: bool = True, precision: str = "fp64",
                  charge_every_step: bool = True, bbox_expansion: float = BBOX_EXPANSION,
-                 steps_per_launch: int = 1):
+                 steps_per_launch: int = 1, sort_interval: int = 20):
         L = _lib.lib()
         self.mesh = mesh
         self._nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
@@ -53,7 +53,7 @@ class ParticleInCellStep:
         g = _lib.GasDesc(gas.mass, gas.omega, gas.alpha, gas.concentration, _lib.MODEL[gas.model], 0)
         prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
         prm = _lib.Params(dt, bbox_expansion, seed, run, int(em_push), prec, int(charge_every_step),
-                          len(species), steps_per_launch, sp, g)
+                          len(species), steps_per_launch, sp, g, int(sort_interval), 0)
         self.ctx = C.c_void_p()
         _lib.check(L.start_create(C.byref(self.ctx), C.byref(md), C.byref(prm)), None)
         self.n = 0

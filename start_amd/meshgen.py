@@ -82,9 +82,21 @@ def _orient_positive(nodes: np.ndarray, tets: np.ndarray) -> np.ndarray:
 def boundary_faces(tets: np.ndarray) -> np.ndarray:
     """Faces used by exactly one tet, in (tet id, local face) order."""
     faces = tets[:, FACES].reshape(-1, 3)
-    key = np.sort(faces, axis=1)
-    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
-    once = cnt[inv.ravel()] == 1
+    s = np.sort(faces, axis=1).astype(np.int64)
+    m = int(tets.max()) + 1
+    if m ** 3 < 2 ** 62:
+        key = (s[:, 0] * m + s[:, 1]) * m + s[:, 2]
+        order = np.argsort(key)
+        ks = key[order]
+        dup = np.zeros(len(ks), dtype=bool)
+        same = ks[1:] == ks[:-1]
+        dup[1:] |= same
+        dup[:-1] |= same
+        once = np.zeros(len(ks), dtype=bool)
+        once[order] = ~dup
+    else:
+        _, inv, cnt = np.unique(s, axis=0, return_inverse=True, return_counts=True)
+        once = cnt[inv.ravel()] == 1
     return faces[once]
 
 

@@ -36,6 +36,7 @@ class Pair:
                                   O.MODEL[g.model]) for s in w.species]
         tet0 = self.om.locate_many(pos[:, 0], pos[:, 1], pos[:, 2])
         self.st = O.State.new(pos, vel, sp, np.arange(n, dtype=np.uint64), tet0)
+        self.tet0 = tet0.copy()
         self.sim = O.Sim(self.om, ks, O.MODEL[g.model], g.concentration, w.dt, w.seed,
                          em_push=em_push, E=self.E)
         self.charge = charge
@@ -76,7 +77,8 @@ class Pair:
         tc = self.eng.tri_counts()
         assert np.array_equal(tc, self.sim.tri_count[: len(tc)]), f"{where}: triangle counts differ"
         res = {"max_rel_x": float(ex.max()), "max_rel_v": float(ev.max()),
-               "settled": int((st.status == 1).sum())}
+               "settled": int((st.status == 1).sum()),
+               "owner_changed": int(((st.tet != self.tet0) & (st.status == 0)).sum())}
         if self.charge:
             qg = self.eng.node_charge()
             err = np.abs(qg - self.qo) / np.maximum(np.abs(self.qo), E_CHARGE)

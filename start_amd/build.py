@@ -31,7 +31,7 @@ def build_library(verbose: bool = False) -> str:
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-munsafe-fp-atomics",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-result", "-I", os.path.join(ROOT, "include"),
+           "-Wno-unused-result", "-Wl,--no-undefined", "-I", os.path.join(ROOT, "include"),
            "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -38,9 +38,11 @@ enum Counter {
 constexpr int kProcSlots = 64;
 
 struct WorkItem {
-    double x0, y0, z0;
-    int32_t i;
-    int32_t t0;
+    double x0, y0, z0;  // absolute start of the deferred step
+    int32_t i;          // particle slot
+    int32_t t0;         // owner at the start of that step
+    int32_t k;          // step index within the launch
+    int32_t pad;
 };
 
 struct DevMesh {
@@ -86,8 +88,8 @@ struct StepParams {
     int32_t nspecies;
     const int64_t* n_active;  // live particles lead the arrays after a sort
     double* zero_charge;      // node-charge buffer of the next step, cleared here
-    int32_t work_slot;        // work-list counter of this step: cnt[kCntWork + 2*slot]
-    int32_t pad;
+    int32_t work_slot;        // work-list counter of this launch: cnt[kCntWork + 2*slot]
+    int32_t nsteps;           // steps fused in this launch (charge accumulates over them)
 };
 
 }  // namespace sg

@@ -135,14 +135,19 @@ __device__ inline R collision_probability(int model, const CollideK<R>& k, R n, 
 }
 
 // Post-collision velocity for variates u2 (cos chi) and u3 (phi).
+// FP64 follows the reference expressions operation by operation.  FP32 uses
+// algebraically identical forms that avoid cancellation: sin(chi) as
+// sqrt((1-c)(1+c)), and the VSS rotation by beta = acos(pz/|p|),
+// gamma = atan2(py, px) written with the direction cosines of p.
 template <typename R>
 __device__ inline void scatter_velocity(int model, const CollideK<R>& k, R u2, R u3, R& vx, R& vy, R& vz)
 {
+    constexpr bool kF64 = sizeof(R) == 8;
     R xi_cos;
     if (model == 1) xi_cos = (u2 * (R(1) - (R(-1)))) + (R(-1));       // rng(-1, 1)
     else if (model == 2) xi_cos = (u2 * (R(1) - R(0))) + R(0);         // rng()
     else xi_cos = R(2) * m_pow<R>(u2, k.inv_alpha) - R(1);             // 2*pow(rng(),1/a)-1
-    R xi_sin = m_sqrt<R>(R(1) - xi_cos * xi_cos);
+    R xi_sin = kF64 ? m_sqrt<R>(R(1) - xi_cos * xi_cos) : m_sqrt<R>((R(1) - xi_cos) * (R(1) + xi_cos));
     R phi = (u3 * (R(2) * R(kPi) - R(0))) + R(0);                       // rng(0, 2*PI)
     R sphi, cphi;
     m_sincos<R>(phi, &sphi, &cphi);
@@ -153,14 +158,27 @@ __device__ inline void scatter_velocity(int model, const CollideK<R>& k, R u2, R
     R dx = x * mp, dy = y * mp, dz = z * mp;
     if (model == 3) {
         // calcBetaGamma + rotation (GeometryVector.hpp:448-474, :66-90)
-        R beta = m_acos<R>(pz / mp), gamma = m_atan2<R>(py, px);
         R sb, cb, sgm, cg;
-        m_sincos<R>(beta, &sb, &cb);
+        if (kF64) {
+            R beta = m_acos<R>(pz / mp), gamma = m_atan2<R>(py, px);
+            m_sincos<R>(beta, &sb, &cb);
+            m_sincos<R>(gamma, &sgm, &cg);
+        } else {
+            R rho = m_sqrt<R>(px * px + py * py);
+            cb = pz / mp;
+            sb = rho / mp;
+            if (rho > R(0)) {
+                cg = px / rho;
+                sgm = py / rho;
+            } else {  // atan2(+-0, px): gamma = 0 or pi
+                cg = px < R(0) ? R(-1) : R(1);
+                sgm = R(0);
+            }
+        }
         R tx = cb * dx + sb * dz;
         R tz = -sb * dx + cb * dz;
         dx = tx;
         dz = tz;
-        m_sincos<R>(gamma, &sgm, &cg);
         tx = cg * dx - sgm * dy;
         R ty = sgm * dx + cg * dy;
         dx = tx;

@@ -77,5 +77,8 @@ template hipError_t sort_particles<double>(hipStream_t, const DevParticles<doubl
                                            double*, double*, double*, int32_t*, int32_t*, int32_t*, uint32_t*,
                                            int32_t*, int32_t*, int32_t*, int32_t*, void*, size_t, int32_t, int,
                                            int64_t*);
+template hipError_t sort_particles<float>(hipStream_t, const DevParticles<float>&, float*, float*, float*, float*,
+                                          float*, float*, int32_t*, int32_t*, int32_t*, uint32_t*, int32_t*,
+                                          int32_t*, int32_t*, int32_t*, void*, size_t, int32_t, int, int64_t*);
 
 }  // namespace sg

@@ -1,6 +1,6 @@
 // C ABI of libstart_gpu (declared in include/start_gpu.h): device state of one
 // simulation context and the host side of the step loop.
-#include "../../include/start_gpu.h"
+#include "start_gpu.h"
 
 #include <math.h>
 #include <string.h>
@@ -19,15 +19,16 @@ template <typename R>
 void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const StepParams& S, const SpeciesK* sp,
                  double* node_charge, WorkItem* wl, unsigned long long* cnt, unsigned long long* tri_count,
                  hipEvent_t ev_mid);
-template <typename R>
-void launch_locate(hipStream_t s, const DevMesh& M, const R* x, const R* y, const R* z, int64_t n, int32_t* tet,
-                   int32_t* flags, unsigned long long* cnt);
+void launch_locate(hipStream_t s, const DevMesh& M, const double* x, const double* y, const double* z, int64_t n,
+                   int32_t* tet, int32_t* flags, float* lx, float* ly, float* lz, unsigned long long* cnt);
 template <typename R>
 void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,
                    double* node_charge);
 void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32);
 void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
+void launch_abs_positions(hipStream_t s, const DevMesh& M, const float* lx, const float* ly, const float* lz,
+                          const int32_t* tet, int64_t n, double* x, double* y, double* z);
 size_t sort_temp_bytes(int64_t n, int nbits);
 template <typename R>
 hipError_t sort_particles(hipStream_t s, const DevParticles<R>& P, R* x, R* y, R* z, R* vx, R* vy, R* vz,
@@ -46,6 +47,13 @@ struct TimedStep {
     hipEvent_t a0, a1, b1;
 };
 
+// one set of particle arrays; positions/velocities are double (FP64) or float (mixed)
+struct PBuf {
+    void *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    int32_t *tet = nullptr, *flags = nullptr, *hit = nullptr;
+    uint32_t* pidx = nullptr;
+};
+
 }  // namespace
 
 struct start_ctx {
@@ -53,10 +61,13 @@ struct start_ctx {
     hipStream_t stream = nullptr;
     HostMesh hm;
     start_params prm{};
+    bool mixed = false;
+    size_t es = sizeof(double);  // bytes of one position/velocity component
     std::vector<SpeciesK> species;
     // device mesh
     double* d_nodes = nullptr;
     double* d_geo = nullptr;
+    float* d_geo32 = nullptr;
     double* d_E = nullptr;
     float* d_E32 = nullptr;
     int4* d_tnode = nullptr;
@@ -65,29 +76,25 @@ struct start_ctx {
     uint8_t* d_tri_deg = nullptr;
     int32_t *d_tgs = nullptr, *d_tgl = nullptr, *d_sgs = nullptr, *d_sgl = nullptr;
     unsigned long long* d_tri_count = nullptr;
-    double* d_node_charge = nullptr;  // two buffers of n_nodes: this step's and the next one's
-    int cb = 0;                       // buffer the next step scatters into
+    double* d_node_charge = nullptr;  // two buffers of n_nodes: this launch's and the next one's
+    int cb = 0;                       // buffer the next launch scatters into
     int latest = 0;                   // buffer holding the most recent node charge
-    int wslot = 0;                    // work-list counter slot of the next step
+    int wslot = 0;                    // work-list counter slot of the next launch
     SpeciesK* d_species = nullptr;
+    float tau32 = 0.0f;
     // particles
     int64_t n = 0;
-    double *d_x = nullptr, *d_y = nullptr, *d_z = nullptr, *d_vx = nullptr, *d_vy = nullptr, *d_vz = nullptr;
-    int32_t *d_tet = nullptr, *d_flags = nullptr, *d_hit = nullptr;
-    uint32_t* d_pidx = nullptr;
-    // second set for the sort, and sort scratch
-    double *d_x2 = nullptr, *d_y2 = nullptr, *d_z2 = nullptr, *d_vx2 = nullptr, *d_vy2 = nullptr, *d_vz2 = nullptr;
-    int32_t *d_tet2 = nullptr, *d_flags2 = nullptr, *d_hit2 = nullptr;
-    uint32_t* d_pidx2 = nullptr;
+    PBuf A, B;  // live arrays and the sort target
+    std::vector<uint64_t> ids;
+    WorkItem* d_wl = nullptr;
     int32_t *d_keys = nullptr, *d_keys_out = nullptr, *d_idx = nullptr, *d_perm = nullptr;
     void* d_sort_temp = nullptr;
     size_t sort_temp = 0;
     int sort_bits = 1;
     int64_t* d_nactive = nullptr;
-    std::vector<uint64_t> ids;
-    WorkItem* d_wl = nullptr;
     unsigned long long* d_cnt = nullptr;
     uint64_t step = 0;
+    uint64_t last_sort = 0;
     // timing
     bool timing = false;
     std::vector<TimedStep> events;
@@ -100,7 +107,7 @@ struct start_ctx {
         M.geo = d_geo;
         M.E = d_E;
         M.E32 = d_E32;
-        M.geo32 = nullptr;
+        M.geo32 = d_geo32;
         M.tnode = d_tnode;
         M.tnbr = d_tnbr;
         M.tris = d_tris;
@@ -114,12 +121,14 @@ struct start_ctx {
         M.n_tets = hm.n_tets;
         M.n_tris = hm.n_tris;
         M.tau = hm.tau;
-        M.tau32 = 0.0f;
+        M.tau32 = tau32;
         return M;
     }
-    DevParticles<double> particles64() const
+    template <typename R>
+    DevParticles<R> view() const
     {
-        return DevParticles<double>{d_x, d_y, d_z, d_vx, d_vy, d_vz, d_tet, d_flags, d_hit, d_pidx, n};
+        return DevParticles<R>{(R*)A.x, (R*)A.y, (R*)A.z, (R*)A.vx, (R*)A.vy, (R*)A.vz,
+                               A.tet,   A.flags, A.hit,  A.pidx,  n};
     }
 };
 
@@ -147,44 +156,58 @@ hipError_t dalloc(T** p, size_t count)
     return hipMalloc((void**)p, sizeof(T) * count);
 }
 
-void free_particles(start_ctx* c)
+hipError_t alloc_pbuf(PBuf& b, int64_t n, size_t es)
 {
-    void* ptrs[] = {c->d_x,     c->d_y,     c->d_z,     c->d_vx,     c->d_vy,       c->d_vz,  c->d_tet,
-                    c->d_flags, c->d_hit,   c->d_pidx,  c->d_wl,     c->d_x2,       c->d_y2,  c->d_z2,
-                    c->d_vx2,   c->d_vy2,   c->d_vz2,   c->d_tet2,   c->d_flags2,   c->d_hit2, c->d_pidx2,
-                    c->d_keys,  c->d_keys_out, c->d_idx, c->d_perm,  c->d_sort_temp};
+    hipError_t e = hipSuccess;
+    void** f[6] = {&b.x, &b.y, &b.z, &b.vx, &b.vy, &b.vz};
+    for (auto p : f)
+        if (e == hipSuccess) e = hipMalloc(p, es * (size_t)std::max<int64_t>(n, 1));
+    if (e == hipSuccess) e = dalloc(&b.tet, n);
+    if (e == hipSuccess) e = dalloc(&b.flags, n);
+    if (e == hipSuccess) e = dalloc(&b.hit, n);
+    if (e == hipSuccess) e = dalloc(&b.pidx, n);
+    return e;
+}
+
+void free_pbuf(PBuf& b)
+{
+    void* ptrs[] = {b.x, b.y, b.z, b.vx, b.vy, b.vz, b.tet, b.flags, b.hit, b.pidx};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    c->d_x = c->d_y = c->d_z = c->d_vx = c->d_vy = c->d_vz = nullptr;
-    c->d_x2 = c->d_y2 = c->d_z2 = c->d_vx2 = c->d_vy2 = c->d_vz2 = nullptr;
-    c->d_tet = c->d_flags = c->d_hit = c->d_tet2 = c->d_flags2 = c->d_hit2 = nullptr;
-    c->d_pidx = c->d_pidx2 = nullptr;
+    b = PBuf{};
+}
+
+void free_particles(start_ctx* c)
+{
+    free_pbuf(c->A);
+    free_pbuf(c->B);
+    void* ptrs[] = {c->d_wl, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    c->d_wl = nullptr;
     c->d_keys = c->d_keys_out = c->d_idx = c->d_perm = nullptr;
     c->d_sort_temp = nullptr;
     c->sort_temp = 0;
-    c->d_wl = nullptr;
     c->n = 0;
 }
 
 // radix sort by owning tet, settled particles last, then swap the buffer sets
+template <typename R>
+hipError_t sort_typed(start_ctx* c)
+{
+    PBuf& B = c->B;
+    return sort_particles<R>(c->stream, c->view<R>(), (R*)B.x, (R*)B.y, (R*)B.z, (R*)B.vx, (R*)B.vy, (R*)B.vz, B.tet,
+                             B.flags, B.hit, B.pidx, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp,
+                             c->sort_temp, (int32_t)c->hm.n_tets, c->sort_bits, c->d_nactive);
+}
+
 int sort_now(start_ctx* c)
 {
-    if (c->n == 0) return START_OK;
-    hipError_t e = sort_particles<double>(c->stream, c->particles64(), c->d_x2, c->d_y2, c->d_z2, c->d_vx2, c->d_vy2,
-                                          c->d_vz2, c->d_tet2, c->d_flags2, c->d_hit2, c->d_pidx2, c->d_keys,
-                                          c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp, c->sort_temp,
-                                          (int32_t)c->hm.n_tets, c->sort_bits, c->d_nactive);
+    if (c->n == 0 || !c->B.x) return START_OK;
+    hipError_t e = c->mixed ? sort_typed<float>(c) : sort_typed<double>(c);
     if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("particle sort: ") + hipGetErrorString(e));
-    std::swap(c->d_x, c->d_x2);
-    std::swap(c->d_y, c->d_y2);
-    std::swap(c->d_z, c->d_z2);
-    std::swap(c->d_vx, c->d_vx2);
-    std::swap(c->d_vy, c->d_vy2);
-    std::swap(c->d_vz, c->d_vz2);
-    std::swap(c->d_tet, c->d_tet2);
-    std::swap(c->d_flags, c->d_flags2);
-    std::swap(c->d_hit, c->d_hit2);
-    std::swap(c->d_pidx, c->d_pidx2);
+    std::swap(c->A, c->B);
+    c->last_sort = c->step;
     return START_OK;
 }
 
@@ -206,6 +229,43 @@ SpeciesK species_constants(const start_species_desc& s, const start_gas_desc& g)
     k.mass_ct = t_mass / (t_mass + p_mass);
     k.inv_alpha = g.alpha != 0.0 ? 1. / g.alpha : 0.0;
     return k;
+}
+
+// FP32 copy of the barycentric maps (A row-major, then v0) and the FP32
+// certification threshold.  For a stored local position l (|l|_inf <= diam):
+// lambda_j = a_j . l with a_j rounded to FP32 and a 3-term FP32 dot product
+// errs by <= (eps32 + gamma_3) |A|_inf diam ~ 4 eps32 R (R = |A|_inf diam);
+// lambda_0 = 1 - ((l1 + l2) + l3) adds the three and gamma_3 (1 + sum|l_j|),
+// so |lambda32 - lambda| <= (12 R + 7) eps32; tau32 takes 1.25x that.
+float build_geo32(const HostMesh& hm, std::vector<float>& g32)
+{
+    const double eps32 = 5.9604644775390625e-08;  // 2^-24
+    g32.assign((size_t)(12 * hm.n_tets), 0.0f);
+    double tau = 0.0;
+    for (int64_t t = 0; t < hm.n_tets; ++t) {
+        const double* g = &hm.geo[12 * (size_t)t];
+        float* o = &g32[12 * (size_t)t];
+        double rowsum = 0.0;
+        for (int r = 0; r < 3; ++r) {
+            double s = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                o[3 * r + q] = (float)g[3 + 3 * r + q];
+                s += fabs(g[3 + 3 * r + q]);
+            }
+            rowsum = std::max(rowsum, s);
+        }
+        o[9] = (float)g[0];
+        o[10] = (float)g[1];
+        o[11] = (float)g[2];
+        double diam = 0.0;
+        const int32_t* v = &hm.tets[4 * (size_t)t];
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b)
+                for (int k = 0; k < 3; ++k)
+                    diam = std::max(diam, fabs(hm.nodes[3 * (size_t)v[a] + k] - hm.nodes[3 * (size_t)v[b] + k]));
+        tau = std::max(tau, 1.25 * eps32 * (12.0 * rowsum * diam + 7.0));
+    }
+    return (float)tau;
 }
 
 int check_kernel_errors(start_ctx* c)
@@ -239,8 +299,8 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         return fail(nullptr, START_ERR_ARG, "unknown precision");
     if (params->gas.model < START_MODEL_NONE || params->gas.model > START_MODEL_VSS)
         return fail(nullptr, START_ERR_ARG, "unknown collision model");
-    if (params->precision == START_PREC_MIXED)
-        return fail(nullptr, START_ERR_ARG, "mixed precision path is not available in this build");
+    if (params->steps_per_launch < 0 || params->sort_interval < 0)
+        return fail(nullptr, START_ERR_ARG, "steps_per_launch and sort_interval must be >= 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(nullptr, START_ERR_HIP, "no HIP device available");
@@ -248,6 +308,9 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
     if (!c) return fail(nullptr, START_ERR_NOMEM, "host allocation failed");
     c->prm = *params;
     c->prm.species = nullptr;
+    if (c->prm.steps_per_launch < 1) c->prm.steps_per_launch = 1;
+    c->mixed = params->precision == START_PREC_MIXED;
+    c->es = c->mixed ? sizeof(float) : sizeof(double);
     double expansion = params->bbox_expansion > 0.0 ? params->bbox_expansion : 1.01;
     std::string e = prepare_mesh(c->hm, mesh->n_nodes, mesh->nodes, mesh->n_tets, mesh->tets, mesh->n_tris,
                                  mesh->tris, expansion);
@@ -256,6 +319,14 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         return fail(nullptr, START_ERR_MESH, e);
     }
     for (int s = 0; s < params->n_species; ++s) c->species.push_back(species_constants(params->species[s], params->gas));
+    std::vector<float> g32;
+    if (c->mixed) {
+        c->tau32 = build_geo32(c->hm, g32);
+        if (!(c->tau32 < 0.01f)) {
+            delete c;
+            return fail(nullptr, START_ERR_MESH, "mesh too ill-conditioned for the FP32 barycentric fast path");
+        }
+    }
 
     auto up = [&](void* dst, const void* src, size_t bytes) {
         return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -275,6 +346,10 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
     chk(dalloc(&c->d_nodes, 3 * M));
     chk(dalloc(&c->d_geo, 12 * T));
     chk(dalloc(&c->d_E, 3 * T));
+    if (c->mixed) {
+        chk(dalloc(&c->d_geo32, 12 * T));
+        chk(dalloc(&c->d_E32, 4 * T));
+    }
     chk(dalloc(&c->d_tnode, T));
     chk(dalloc(&c->d_tnbr, T));
     chk(dalloc(&c->d_tris, 3 * S));
@@ -292,6 +367,10 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         chk(up(c->d_nodes, hm.nodes.data(), sizeof(double) * 3 * M));
         chk(up(c->d_geo, hm.geo.data(), sizeof(double) * 12 * T));
         chk(hipMemset(c->d_E, 0, sizeof(double) * 3 * T));
+        if (c->mixed) {
+            chk(up(c->d_geo32, g32.data(), sizeof(float) * 12 * T));
+            chk(hipMemset(c->d_E32, 0, sizeof(float) * 4 * T));
+        }
         chk(up(c->d_tnode, tn.data(), sizeof(int4) * T));
         chk(up(c->d_tnbr, tb.data(), sizeof(int4) * T));
         if (S) chk(up(c->d_tris, hm.tris.data(), sizeof(int32_t) * 3 * S));
@@ -319,9 +398,9 @@ void start_destroy(start_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_particles(c);
-    void* ptrs[] = {c->d_nodes, c->d_geo, c->d_E, c->d_E32, c->d_tnode, c->d_tnbr, c->d_tris, c->d_tri_deg,
-                    c->d_tgs, c->d_tgl, c->d_sgs, c->d_sgl, c->d_tri_count, c->d_node_charge, c->d_species, c->d_cnt,
-                    c->d_nactive};
+    void* ptrs[] = {c->d_nodes, c->d_geo, c->d_geo32, c->d_E, c->d_E32, c->d_tnode, c->d_tnbr, c->d_tris,
+                    c->d_tri_deg, c->d_tgs, c->d_tgl, c->d_sgs, c->d_sgl, c->d_tri_count, c->d_node_charge,
+                    c->d_species, c->d_cnt, c->d_nactive};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->events) {
@@ -348,61 +427,24 @@ int start_load_particles(start_ctx* c, int64_t n, const double* x, const double*
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     free_particles(c);
-    hipError_t he = hipSuccess;
-    auto chk = [&](hipError_t e) {
-        if (he == hipSuccess && e != hipSuccess) he = e;
-    };
-    chk(dalloc(&c->d_x, n));
-    chk(dalloc(&c->d_y, n));
-    chk(dalloc(&c->d_z, n));
-    chk(dalloc(&c->d_vx, n));
-    chk(dalloc(&c->d_vy, n));
-    chk(dalloc(&c->d_vz, n));
-    chk(dalloc(&c->d_tet, n));
-    chk(dalloc(&c->d_flags, n));
-    chk(dalloc(&c->d_hit, n));
-    chk(dalloc(&c->d_pidx, n));
-    chk(dalloc(&c->d_wl, n));
-    if (c->prm.sort_interval > 0) {
-        chk(dalloc(&c->d_x2, n));
-        chk(dalloc(&c->d_y2, n));
-        chk(dalloc(&c->d_z2, n));
-        chk(dalloc(&c->d_vx2, n));
-        chk(dalloc(&c->d_vy2, n));
-        chk(dalloc(&c->d_vz2, n));
-        chk(dalloc(&c->d_tet2, n));
-        chk(dalloc(&c->d_flags2, n));
-        chk(dalloc(&c->d_hit2, n));
-        chk(dalloc(&c->d_pidx2, n));
-        chk(dalloc(&c->d_keys, n));
-        chk(dalloc(&c->d_keys_out, n));
-        chk(dalloc(&c->d_idx, n));
-        chk(dalloc(&c->d_perm, n));
+    hipError_t he = alloc_pbuf(c->A, n, c->es);
+    if (he == hipSuccess) he = dalloc(&c->d_wl, n);
+    if (he == hipSuccess && c->prm.sort_interval > 0) {
+        he = alloc_pbuf(c->B, n, c->es);
+        if (he == hipSuccess) he = dalloc(&c->d_keys, n);
+        if (he == hipSuccess) he = dalloc(&c->d_keys_out, n);
+        if (he == hipSuccess) he = dalloc(&c->d_idx, n);
+        if (he == hipSuccess) he = dalloc(&c->d_perm, n);
         int bits = 1;
         while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
         c->sort_bits = bits;
         c->sort_temp = sort_temp_bytes(n, bits);
-        chk(hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16)));
+        if (he == hipSuccess) he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
     }
     if (he != hipSuccess) {
         free_particles(c);
         return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
                     std::string("particle allocation failed: ") + hipGetErrorString(he));
-    }
-    std::vector<uint32_t> pidx((size_t)n);
-    for (int64_t i = 0; i < n; ++i) pidx[(size_t)i] = (uint32_t)i;
-    std::vector<int32_t> hit((size_t)n, -1);
-    size_t b = sizeof(double) * (size_t)n;
-    if (n > 0) {
-        HIPCHECK(c, hipMemcpy(c->d_x, x, b, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_y, y, b, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_z, z, b, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_vx, vx, b, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_vy, vy, b, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_vz, vz, b, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_flags, flags.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_hit, hit.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->d_pidx, pidx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     }
     c->n = n;
     c->ids.assign(ids ? ids : (const uint64_t*)nullptr, ids ? ids + n : (const uint64_t*)nullptr);
@@ -413,20 +455,65 @@ int start_load_particles(start_ctx* c, int64_t n, const double* x, const double*
     HIPCHECK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_tri_count, 0, sizeof(unsigned long long) * std::max<int64_t>(c->hm.n_tris, 1),
                                c->stream));
-    int64_t nact = n;
-    HIPCHECK(c, hipMemcpyAsync(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_node_charge, 0, sizeof(double) * 2 * c->hm.n_nodes, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->cb = 0;
     c->latest = 0;
     c->wslot = 0;
+    c->step = 0;
+    c->last_sort = 0;
+    if (n == 0) return START_OK;
+
+    std::vector<uint32_t> pidx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) pidx[(size_t)i] = (uint32_t)i;
+    std::vector<int32_t> hit((size_t)n, -1);
+    size_t b8 = sizeof(double) * (size_t)n;
+    HIPCHECK(c, hipMemcpy(c->A.flags, flags.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->A.hit, hit.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->A.pidx, pidx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    int64_t nact = n;
+    HIPCHECK(c, hipMemcpy(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice));
     DevMesh M = c->mesh_view();
-    launch_locate<double>(c->stream, M, c->d_x, c->d_y, c->d_z, n, c->d_tet, c->d_flags, c->d_cnt);
+    if (!c->mixed) {
+        HIPCHECK(c, hipMemcpy(c->A.x, x, b8, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->A.y, y, b8, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->A.z, z, b8, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->A.vx, vx, b8, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->A.vy, vy, b8, hipMemcpyHostToDevice));
+        HIPCHECK(c, hipMemcpy(c->A.vz, vz, b8, hipMemcpyHostToDevice));
+        launch_locate(c->stream, M, (double*)c->A.x, (double*)c->A.y, (double*)c->A.z, n, c->A.tet, c->A.flags,
+                      nullptr, nullptr, nullptr, c->d_cnt);
+    } else {
+        // positions go to the frame of their owner; velocities to FP32
+        double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+        hipError_t e = dalloc(&dx, n);
+        if (e == hipSuccess) e = dalloc(&dy, n);
+        if (e == hipSuccess) e = dalloc(&dz, n);
+        if (e == hipSuccess) e = hipMemcpy(dx, x, b8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dy, y, b8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dz, z, b8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_locate(c->stream, M, dx, dy, dz, n, c->A.tet, c->A.flags, (float*)c->A.x, (float*)c->A.y,
+                          (float*)c->A.z, c->d_cnt);
+            e = hipStreamSynchronize(c->stream);
+        }
+        (void)hipFree(dx);
+        (void)hipFree(dy);
+        (void)hipFree(dz);
+        if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("particle upload: ") + hipGetErrorString(e));
+        std::vector<float> f((size_t)n);
+        const double* vs[3] = {vx, vy, vz};
+        void* vd[3] = {c->A.vx, c->A.vy, c->A.vz};
+        for (int k = 0; k < 3; ++k) {
+            for (int64_t i = 0; i < n; ++i) f[(size_t)i] = (float)vs[k][i];
+            HIPCHECK(c, hipMemcpy(vd[k], f.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        }
+    }
     HIPCHECK(c, hipGetLastError());
     if (c->prm.sort_interval > 0) {
         int rc = sort_now(c);
         if (rc != START_OK) return rc;
     }
-    c->step = 0;
     return check_kernel_errors(c);
 }
 
@@ -434,6 +521,8 @@ int start_set_tet_field(start_ctx* c, const double* E)
 {
     if (!c || !E) return fail(c, START_ERR_ARG, "null argument");
     HIPCHECK(c, hipMemcpyAsync(c->d_E, E, sizeof(double) * 3 * c->hm.n_tets, hipMemcpyHostToDevice, c->stream));
+    if (c->d_E32) launch_e32(c->stream, c->d_E, c->d_E32, c->hm.n_tets);
+    HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return START_OK;
 }
@@ -458,9 +547,11 @@ int start_step(start_ctx* c, int32_t nsteps)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
     if (nsteps < 0) return fail(c, START_ERR_ARG, "negative step count");
-    if (!c->d_x) return fail(c, START_ERR_STATE, "no particles loaded");
+    if (c->n == 0) {
+        c->step += (uint64_t)nsteps;
+        return START_OK;
+    }
     DevMesh M = c->mesh_view();
-    DevParticles<double> P = c->particles64();
     StepParams S{};
     S.dt = c->prm.dt;
     S.conc = c->prm.gas.concentration;
@@ -476,16 +567,19 @@ int start_step(start_ctx* c, int32_t nsteps)
     S.nspecies = (int32_t)c->species.size();
     S.n_active = c->d_nactive;
     const int64_t Mn = c->hm.n_nodes;
-    for (int32_t k = 0; k < nsteps; ++k) {
+    const int32_t K = c->prm.steps_per_launch;
+    int32_t done = 0;
+    while (done < nsteps) {
+        if (c->prm.sort_interval > 0 && c->step - c->last_sort >= (uint64_t)c->prm.sort_interval) {
+            int rc = sort_now(c);
+            if (rc != START_OK) return rc;
+        }
+        int32_t kk = std::min(K, nsteps - done);
         double* nc = c->prm.charge_every_step ? c->d_node_charge + (size_t)c->cb * Mn : nullptr;
         S.zero_charge = c->prm.charge_every_step ? c->d_node_charge + (size_t)(1 - c->cb) * Mn : nullptr;
         S.work_slot = c->wslot;
-        if (c->prm.sort_interval > 0 && c->step > 0 && c->step % (uint64_t)c->prm.sort_interval == 0) {
-            int rc = sort_now(c);
-            if (rc != START_OK) return rc;
-            P = c->particles64();
-        }
         S.step = (uint32_t)c->step;
+        S.nsteps = kk;
         TimedStep* ev = nullptr;
         if (c->timing) {
             if (c->ev_used == c->events.size()) {
@@ -498,11 +592,16 @@ int start_step(start_ctx* c, int32_t nsteps)
             ev = &c->events[c->ev_used++];
         }
         if (ev) HIPCHECK(c, hipEventRecord(ev->a0, c->stream));
-        launch_step<double>(c->stream, M, P, S, c->d_species, nc, c->d_wl, c->d_cnt, c->d_tri_count,
-                            ev ? ev->a1 : nullptr);
+        if (c->mixed)
+            launch_step<float>(c->stream, M, c->view<float>(), S, c->d_species, nc, c->d_wl, c->d_cnt,
+                               c->d_tri_count, ev ? ev->a1 : nullptr);
+        else
+            launch_step<double>(c->stream, M, c->view<double>(), S, c->d_species, nc, c->d_wl, c->d_cnt,
+                                c->d_tri_count, ev ? ev->a1 : nullptr);
         if (ev) HIPCHECK(c, hipEventRecord(ev->b1, c->stream));
         HIPCHECK(c, hipGetLastError());
-        c->step++;
+        c->step += (uint64_t)kk;
+        done += kk;
         c->wslot ^= 1;
         if (nc) {
             c->latest = c->cb;
@@ -515,11 +614,12 @@ int start_step(start_ctx* c, int32_t nsteps)
 int start_gather_charge(start_ctx* c)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
-    if (!c->d_x) return fail(c, START_ERR_STATE, "no particles loaded");
+    if (c->n == 0) return START_OK;
     const int64_t Mn = c->hm.n_nodes;
     double* cur = c->d_node_charge + (size_t)c->cb * Mn;
     launch_zero_charge(c->stream, cur, c->d_node_charge + (size_t)(1 - c->cb) * Mn, Mn);
-    launch_gather<double>(c->stream, c->mesh_view(), c->particles64(), c->d_species, cur);
+    if (c->mixed) launch_gather<float>(c->stream, c->mesh_view(), c->view<float>(), c->d_species, cur);
+    else launch_gather<double>(c->stream, c->mesh_view(), c->view<double>(), c->d_species, cur);
     HIPCHECK(c, hipGetLastError());
     c->latest = c->cb;
     c->cb ^= 1;
@@ -544,18 +644,45 @@ int start_read_particles(start_ctx* c, double* x, double* y, double* z, double* 
     hipStream_t s = c->stream;
     // device order differs from load order after a sort: bring back by load index
     std::vector<uint32_t> pidx((size_t)n);
-    HIPCHECK(c, hipMemcpyAsync(pidx.data(), c->d_pidx, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+    HIPCHECK(c, hipMemcpyAsync(pidx.data(), c->A.pidx, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
     std::vector<double> tmp((size_t)n);
+    std::vector<float> ftmp((size_t)n);
     std::vector<int32_t> itmp((size_t)n);
-    double* dsrc[6] = {c->d_x, c->d_y, c->d_z, c->d_vx, c->d_vy, c->d_vz};
-    double* ddst[6] = {x, y, z, vx, vy, vz};
-    for (int k = 0; k < 6; ++k) {
-        if (!ddst[k]) continue;
-        HIPCHECK(c, hipMemcpyAsync(tmp.data(), dsrc[k], sizeof(double) * n, hipMemcpyDeviceToHost, s));
-        HIPCHECK(c, hipStreamSynchronize(s));
-        for (int64_t j = 0; j < n; ++j) ddst[k][pidx[(size_t)j]] = tmp[(size_t)j];
+    double* pos_src[3] = {nullptr, nullptr, nullptr};
+    double* abs_buf = nullptr;
+    if (c->mixed && (x || y || z)) {
+        HIPCHECK(c, dalloc(&abs_buf, 3 * n));
+        launch_abs_positions(s, c->mesh_view(), (float*)c->A.x, (float*)c->A.y, (float*)c->A.z, c->A.tet, n, abs_buf,
+                             abs_buf + n, abs_buf + 2 * n);
+        pos_src[0] = abs_buf;
+        pos_src[1] = abs_buf + n;
+        pos_src[2] = abs_buf + 2 * n;
+    } else if (!c->mixed) {
+        pos_src[0] = (double*)c->A.x;
+        pos_src[1] = (double*)c->A.y;
+        pos_src[2] = (double*)c->A.z;
     }
-    int32_t* isrc[3] = {c->d_tet, c->d_flags, c->d_hit};
+    double* ddst[6] = {x, y, z, vx, vy, vz};
+    void* vsrc[3] = {c->A.vx, c->A.vy, c->A.vz};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 6 && e == hipSuccess; ++k) {
+        if (!ddst[k]) continue;
+        if (k < 3 || !c->mixed) {
+            const void* src = k < 3 ? (const void*)pos_src[k] : vsrc[k - 3];
+            e = hipMemcpyAsync(tmp.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e == hipSuccess)
+                for (int64_t j = 0; j < n; ++j) ddst[k][pidx[(size_t)j]] = tmp[(size_t)j];
+        } else {
+            e = hipMemcpyAsync(ftmp.data(), vsrc[k - 3], sizeof(float) * n, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e == hipSuccess)
+                for (int64_t j = 0; j < n; ++j) ddst[k][pidx[(size_t)j]] = (double)ftmp[(size_t)j];
+        }
+    }
+    if (abs_buf) (void)hipFree(abs_buf);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("read particles: ") + hipGetErrorString(e));
+    int32_t* isrc[3] = {c->A.tet, c->A.flags, c->A.hit};
     int32_t* idst[3] = {tet, status, hit_tri};
     for (int k = 0; k < 3; ++k) {
         if (!idst[k]) continue;
@@ -587,8 +714,8 @@ int start_read_settled(start_ctx* c, uint64_t* ids, int64_t* n)
     std::vector<int32_t> flags((size_t)c->n);
     std::vector<uint32_t> pidx((size_t)c->n);
     if (c->n) {
-        HIPCHECK(c, hipMemcpyAsync(flags.data(), c->d_flags, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(c, hipMemcpyAsync(pidx.data(), c->d_pidx, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(flags.data(), c->A.flags, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipMemcpyAsync(pidx.data(), c->A.pidx, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     std::vector<uint8_t> settled((size_t)c->n, 0);
@@ -644,7 +771,7 @@ int start_locate(start_ctx* c, int64_t n, const double* x, const double* y, cons
     if (e == hipSuccess) e = hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dz, z, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        launch_locate<double>(c->stream, c->mesh_view(), dx, dy, dz, n, dt, nullptr, c->d_cnt);
+        launch_locate(c->stream, c->mesh_view(), dx, dy, dz, n, dt, nullptr, nullptr, nullptr, nullptr, c->d_cnt);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out_tet, dt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream);

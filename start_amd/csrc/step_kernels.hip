@@ -8,6 +8,8 @@
 //                         grid search for degenerate cases, charge.
 // The rare exact work (tet crossings, faces, hits) is compacted into a work
 // list so the streaming kernel stays short and free of divergent branches.
+#include <type_traits>
+
 #include "kernels.h"
 #include "predicates.h"
 
@@ -173,13 +175,66 @@ struct Resolved {
     int32_t err;
 };
 
-__device__ Resolved resolve_exhaustive(const DevMesh& M, const double* x0, const double* x1)
+// Owner of p starting from a nearby tet: visibility walk (move across a face
+// whose plane separates p from the tet) until p is strictly inside a tet,
+// which is then the unique (so minimum-id) closed-containing tet.  A point on
+// a face/edge/vertex, a walk that reaches the boundary or runs long goes to
+// the grid search, which applies the minimum-id rule directly.
+__device__ int32_t locate_near(const DevMesh& M, int32_t hint, const double* p, int* strict, int* err)
+{
+    int32_t t = hint, prev = -1;
+    for (int it = 0; t >= 0 && it < 64; ++it) {
+        int4 nv = M.tnode[t];
+        double P[4][3];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) load_node(M, nth(nv, k), P[k]);
+        int s[4], nneg = 0, nzero = 0;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            s[f] = orient3d(P[kFace[f][0]], P[kFace[f][1]], P[kFace[f][2]], p);
+            if (s[f] == 2) {
+                *err = 1;
+                s[f] = 0;
+            }
+            nneg += s[f] < 0;
+            nzero += s[f] == 0;
+        }
+        if (nneg == 0) {
+            if (nzero == 0) {
+                *strict = 1;
+                return t;
+            }
+            break;  // on the boundary of t: ties need the grid's minimum-id rule
+        }
+        // cross a separating face, preferring one that does not lead back
+        int4 nb = M.tnbr[t];
+        int32_t next = -1, back = -1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int f = (q + it) & 3;
+            int32_t c = nth(nb, f);
+            if (s[f] >= 0 || c < 0) continue;
+            if (c != prev) {
+                if (next < 0) next = c;
+            } else {
+                back = c;
+            }
+        }
+        if (next < 0) next = back;
+        if (next < 0) break;  // separated by a boundary face: outside or non-convex
+        prev = t;
+        t = next;
+    }
+    return locate_grid(M, p, strict, err);
+}
+
+__device__ Resolved resolve_exhaustive(const DevMesh& M, const double* x0, const double* x1, int32_t hint)
 {
     Resolved r{-1, -1, 0, 1, 0};
     r.hit = first_hit_grid(M, x0, x1, &r.err);
     if (r.hit < 0) {
         int strict = 0;
-        r.tet = locate_grid(M, x1, &strict, &r.err);
+        r.tet = hint >= 0 ? locate_near(M, hint, x1, &strict, &r.err) : locate_grid(M, x1, &strict, &r.err);
         r.strict = strict;
     }
     return r;
@@ -189,9 +244,10 @@ __device__ Resolved resolve_exhaustive(const DevMesh& M, const double* x0, const
 // Generic crossings are decided with exact signs; any zero sign (the segment
 // meets an edge, a vertex or ends on a face) falls back to the grid search,
 // which applies the reference rules directly.
-__device__ Resolved resolve(const DevMesh& M, const double* x0, const double* x1, int32_t t0, bool strict0)
+__device__ __noinline__ Resolved resolve(const DevMesh& M, const double* x0, const double* x1, int32_t t0,
+                                        bool strict0)
 {
-    if (t0 < 0 || !strict0) return resolve_exhaustive(M, x0, x1);
+    if (t0 < 0 || !strict0) return resolve_exhaustive(M, x0, x1, t0);
     int32_t t = t0;
     int entry = -1;
     for (int iter = 0; iter < 256; ++iter) {
@@ -202,13 +258,13 @@ __device__ Resolved resolve(const DevMesh& M, const double* x0, const double* x1
         int neg = 0, zero = 0;
         for (int f = 0; f < 4; ++f) {
             s[f] = orient3d(P[kFace[f][0]], P[kFace[f][1]], P[kFace[f][2]], x1);
-            if (s[f] == 2) return resolve_exhaustive(M, x0, x1);
+            if (s[f] == 2) return resolve_exhaustive(M, x0, x1, t0);
             neg += s[f] < 0;
             zero += s[f] == 0;
         }
         if (neg == 0) {
             if (zero == 0) return Resolved{-1, t, 1, 0, 0};
-            return resolve_exhaustive(M, x0, x1);  // x1 on a face/edge/vertex of t
+            return resolve_exhaustive(M, x0, x1, t0);  // x1 on a face/edge/vertex of t
         }
         int exit_f = -1, nexit = 0;
         for (int f = 0; f < 4; ++f) {
@@ -218,13 +274,13 @@ __device__ Resolved resolve(const DevMesh& M, const double* x0, const double* x1
             const double* c = P[kFace[f][2]];
             int o1 = orient3d(x0, x1, a, b), o2 = orient3d(x0, x1, b, c), o3 = orient3d(x0, x1, c, a);
             if (o1 == 0 || o2 == 0 || o3 == 0 || o1 == 2 || o2 == 2 || o3 == 2)
-                return resolve_exhaustive(M, x0, x1);
+                return resolve_exhaustive(M, x0, x1, t0);
             if (o1 == o2 && o2 == o3) {
                 exit_f = f;
                 ++nexit;
             }
         }
-        if (nexit != 1) return resolve_exhaustive(M, x0, x1);
+        if (nexit != 1) return resolve_exhaustive(M, x0, x1, t0);
         int4 nb = M.tnbr[t];
         int32_t code = nth(nb, exit_f);
         if (code <= -2) return Resolved{-2 - code, -1, 0, 0, 0};  // surface triangle crossed first
@@ -235,7 +291,7 @@ __device__ Resolved resolve(const DevMesh& M, const double* x0, const double* x1
             if (nth(back, g) == t) entry = g;
         t = code;
     }
-    return resolve_exhaustive(M, x0, x1);
+    return resolve_exhaustive(M, x0, x1, t0);
 }
 
 // ---------------------------------------------------------------------------
@@ -291,6 +347,317 @@ __device__ inline void lds_scatter4(int32_t* hk, double* hv, double* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// per-thread tet cache and the two arithmetic layouts
+//   FP64:  position absolute (double), map = v0 + A (double), E (double)
+//   mixed: position relative to v0 of the owning tet (float), map A (float),
+//          E (float); absolute coordinates are v0 (double) + local
+// ---------------------------------------------------------------------------
+// lazily loaded pieces of the owning tet's record (keeps registers free across
+// the collision code)
+__device__ inline void load_E(const DevMesh& M, int32_t t, double& ex, double& ey, double& ez)
+{
+    const double* E = M.E + 3 * (int64_t)t;
+    ex = E[0];
+    ey = E[1];
+    ez = E[2];
+}
+__device__ inline void load_E(const DevMesh& M, int32_t t, float& ex, float& ey, float& ez)
+{
+    float4 e = reinterpret_cast<const float4*>(M.E32)[t];
+    ex = e.x;
+    ey = e.y;
+    ez = e.z;
+}
+__device__ inline void bary_at(const DevMesh& M, int32_t t, double x, double y, double z, double lam[4])
+{
+    bary64(M.geo + 12 * (int64_t)t, x, y, z, lam);
+}
+__device__ inline void bary_at(const DevMesh& M, int32_t t, float x, float y, float z, double lam[4])
+{
+    const float4* a = reinterpret_cast<const float4*>(M.geo32) + 3 * (int64_t)t;
+    float4 a0 = a[0], a1 = a[1], a2 = a[2];
+    float l1 = (a0.x * x + a0.y * y) + a0.z * z;
+    float l2 = (a0.w * x + a1.x * y) + a1.y * z;
+    float l3 = (a1.z * x + a1.w * y) + a2.x * z;
+    float l0 = 1.0f - ((l1 + l2) + l3);
+    lam[0] = l0;
+    lam[1] = l1;
+    lam[2] = l2;
+    lam[3] = l3;
+}
+// bbox test of the frozen rule on the stored position
+__device__ inline bool in_bbox_at(const DevMesh& M, const StepParams& S, int32_t t, double x, double y, double z)
+{
+    return x >= S.lo[0] && x <= S.hi[0] && y >= S.lo[1] && y <= S.hi[1] && z >= S.lo[2] && z <= S.hi[2];
+}
+__device__ inline bool in_bbox_at(const DevMesh& M, const StepParams& S, int32_t t, float x, float y, float z)
+{
+    float ax = x, ay = y, az = z;
+    if (t >= 0) {
+        float4 a2 = (reinterpret_cast<const float4*>(M.geo32) + 3 * (int64_t)t)[2];
+        ax += a2.y;
+        ay += a2.z;
+        az += a2.w;
+    }
+    return ax >= (float)S.lo[0] && ax <= (float)S.hi[0] && ay >= (float)S.lo[1] && ay <= (float)S.hi[1] &&
+           az >= (float)S.lo[2] && az <= (float)S.hi[2];
+}
+
+template <typename R>
+__device__ inline double tau_of(const DevMesh& M);
+template <>
+__device__ inline double tau_of<double>(const DevMesh& M) { return M.tau; }
+template <>
+__device__ inline double tau_of<float>(const DevMesh& M) { return (double)M.tau32; }
+
+__device__ inline void v0_of(const DevMesh& M, int32_t t, double* v0)
+{
+    const double* g = M.geo + 12 * (int64_t)t;
+    v0[0] = g[0];
+    v0[1] = g[1];
+    v0[2] = g[2];
+}
+
+// absolute position of a stored state
+__device__ inline void abs_pos(const DevMesh& M, int32_t t, double x, double y, double z, double* p)
+{
+    p[0] = x;
+    p[1] = y;
+    p[2] = z;
+}
+__device__ inline void abs_pos(const DevMesh& M, int32_t t, float x, float y, float z, double* p)
+{
+    double v0[3] = {0.0, 0.0, 0.0};
+    if (t >= 0) v0_of(M, t, v0);
+    p[0] = v0[0] + (double)x;
+    p[1] = v0[1] + (double)y;
+    p[2] = v0[2] + (double)z;
+}
+
+// store an absolute position in the frame of tet t
+__device__ inline void to_frame(const DevMesh& M, int32_t t, const double* p, double& x, double& y, double& z)
+{
+    x = p[0];
+    y = p[1];
+    z = p[2];
+}
+__device__ inline void to_frame(const DevMesh& M, int32_t t, const double* p, float& x, float& y, float& z)
+{
+    double v0[3] = {0.0, 0.0, 0.0};
+    if (t >= 0) v0_of(M, t, v0);
+    x = (float)(p[0] - v0[0]);
+    y = (float)(p[1] - v0[1]);
+    z = (float)(p[2] - v0[2]);
+}
+
+// ---------------------------------------------------------------------------
+// charge sinks: LDS hash (streaming kernel) or direct atomics (rare paths)
+// ---------------------------------------------------------------------------
+struct LdsSink {
+    int32_t* hk;
+    double* hv;
+    double* nc;
+    __device__ void add4(const int4& nv, double q, const double lam[4]) { lds_scatter4(hk, hv, nc, nv, q, lam); }
+};
+struct GlobalSink {
+    double* nc;
+    __device__ void add4(const int4& nv, double q, const double lam[4]) { scatter4(nc, nv, q, lam); }
+};
+
+// Collision decision u1 < P (HSModel.cpp:15-19, VHSModel.cpp:60-70).  For
+// VHS/VSS the FP64 probability needs pow() every step; an FP32 estimate with a
+// relative error far below 1e-4 decides all draws outside a +-1e-4 band
+// around it, and only draws inside the band evaluate the FP64 expression, so
+// the FP64 decision is reproduced exactly at FP32 cost.
+__device__ __noinline__ bool collides_exact64(int model, const SpeciesK& k, double conc, double dt, double vm,
+                                              double u1)
+{
+    CollideK<double> ck{k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, k.mass_cp, k.mass_ct, k.inv_alpha};
+    return u1 < collision_probability<double>(model, ck, conc, dt, vm);
+}
+
+template <typename R>
+__device__ inline bool collides(const StepParams& S, const SpeciesK& k, R vx, R vy, R vz, double u1)
+{
+    R vm = m_sqrt<R>(vx * vx + vy * vy + vz * vz);
+    if (S.model == 1) {  // HS: no transcendental, evaluate directly
+        CollideK<R> ck{(R)k.sigma_hs, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0};
+        return (R)u1 < collision_probability<R>(1, ck, (R)S.conc, (R)S.dt, vm);
+    }
+    float vmf = (float)vm;
+    CollideK<float> cf{(float)k.sigma_hs, (float)k.vhs_c, (float)k.two_kt, (float)k.m_r, (float)k.vhs_exp,
+                       0.0f, 0.0f, 0.0f};
+    float p32 = collision_probability<float>(S.model, cf, (float)S.conc, (float)S.dt, vmf);
+    if (!std::is_same<R, double>::value) return (float)u1 < p32;
+    double p = (double)p32;
+    if (!(p > 1e-30 && p < 1e30))  // outside the range the FP32 bound covers
+        return collides_exact64(S.model, k, S.conc, S.dt, (double)vm, u1);
+    if (u1 < p * (1.0 - 1e-4)) return true;
+    if (u1 >= p * (1.0 + 1e-4)) return false;
+    return collides_exact64(S.model, k, S.conc, S.dt, (double)vm, u1);
+}
+
+// post-collision velocity (rare: out of line to keep the streaming kernel lean)
+template <typename R>
+__device__ __noinline__ void collide_velocity(int model, const SpeciesK& k, double u2, double u3, R& vx, R& vy, R& vz)
+{
+    CollideK<R> ck{(R)k.sigma_hs, (R)k.vhs_c, (R)k.two_kt, (R)k.m_r, (R)k.vhs_exp,
+                   (R)k.mass_cp, (R)k.mass_ct, (R)k.inv_alpha};
+    scatter_velocity<R>(model, ck, (R)u2, (R)u3, vx, vy, vz);
+}
+
+enum StepOutcome { kDone = 0, kSettledNow = 1, kDeferNow = 2, kFrozenNow = 3 };
+
+template <typename R>
+struct PState {
+    R x, y, z, vx, vy, vz;
+    int32_t t, f;
+};
+
+// One step of one particle (ParticleDynamicsProcessor.cpp:166-199 stage order,
+// then the relocation and charge of the next gather).  kExact: resolve every
+// relocation here (the straggler kernel); otherwise uncertifiable relocations
+// return kDeferNow with the state at x1 and the start position in x0abs.
+// The owning tet's data is re-read each step (L1/L2 hits while the particle
+// stays in it) to keep the register footprint of the streaming kernel small.
+template <typename R, bool kExact, typename Sink>
+__device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, const SpeciesK& k, uint32_t pidx,
+                                        uint32_t step, int remaining, PState<R>& s, Sink& sink, int32_t& hit_tri,
+                                        double* x0abs, int& exh, int& err)
+{
+    const double tau = tau_of<R>(M);
+    // 2. |v| < 10 m/s and 3. outside the expanded bbox: frozen this step and
+    //    every later one (nothing changes its state); it still carries charge
+    R vmod = m_sqrt<R>(s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    bool frozen = vmod < R(10) || !in_bbox_at(M, S, s.t, s.x, s.y, s.z);
+    double lam[4];
+    if (frozen) {
+        if (S.charge && s.t >= 0 && k.charge != 0.0) {
+            bary_at(M, s.t, s.x, s.y, s.z, lam);
+            sink.add4(M.tnode[s.t], k.charge * (double)remaining, lam);
+        }
+        return kFrozenNow;
+    }
+    R dt = (R)S.dt;
+    // 4. push with the owning tet's field (ParticleDynamicsProcessor.cpp:179-180)
+    if (S.em_push && s.t >= 0) {
+        R ex, ey, ez;
+        load_E(M, s.t, ex, ey, ez);
+        push<R>(ex, ey, ez, (R)k.charge, (R)k.mass, dt, s.vx, s.vy, s.vz);
+    }
+    // 6. move (Particle.cpp:157-165)
+    R x1 = s.x + s.vx * dt, y1 = s.y + s.vy * dt, z1 = s.z + s.vz * dt;
+    if (x1 == s.x && y1 == s.y && z1 == s.z) {  // degenerate segment (:188-190)
+        if (S.charge && s.t >= 0 && k.charge != 0.0) {
+            bary_at(M, s.t, s.x, s.y, s.z, lam);
+            sink.add4(M.tnode[s.t], k.charge, lam);
+        }
+        return kDone;
+    }
+    // 7. gas collision (ParticlePhysicsUpdater.cpp:20-24)
+    if (S.model != 0) {
+        double u1, u1b;
+        variates(S.seed, S.run, pidx, step, 0u, u1, u1b);
+        if (collides<R>(S, k, s.vx, s.vy, s.vz, u1)) {
+            double u2, u3;
+            variates(S.seed, S.run, pidx, step, 1u, u2, u3);
+            collide_velocity<R>(S.model, k, u2, u3, s.vx, s.vy, s.vz);
+        }
+    }
+    // 8. relocation and surface hit on the segment x -> x1.  Certified cases:
+    //    x1 strictly inside the owner, or strictly outside exactly one face f
+    //    and strictly inside the other three (the segment leaves through the
+    //    open face f).  The rest needs the exact walk.
+    bool certified = false;
+    if (s.t >= 0 && (s.f & kStrict)) {
+        bary_at(M, s.t, x1, y1, z1, lam);
+        int npos = 0, nneg = 0, fneg = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            npos += lam[q] > tau;
+            if (lam[q] < -tau) {
+                ++nneg;
+                fneg = q;
+            }
+        }
+        if (npos == 4) {
+            certified = true;
+        } else if (npos == 3 && nneg == 1) {
+            int32_t code = nth(M.tnbr[s.t], fneg);
+            if (code <= -2) {
+                hit_tri = -2 - code;  // crosses this surface triangle first
+                s.x = x1;
+                s.y = y1;
+                s.z = z1;
+                return kSettledNow;
+            }
+            // to the neighbour (or out through a face with no triangle)
+            double p1[3];
+            abs_pos(M, s.t, x1, y1, z1, p1);
+            R nx, ny, nz;
+            to_frame(M, code, p1, nx, ny, nz);
+            if (code == -1) {
+                s.t = -1;
+                s.f &= ~kStrict;
+                s.x = nx;
+                s.y = ny;
+                s.z = nz;
+                return kDone;
+            }
+            bary_at(M, code, nx, ny, nz, lam);
+            if (fmin(fmin(lam[0], lam[1]), fmin(lam[2], lam[3])) > tau) {
+                s.t = code;
+                x1 = nx;
+                y1 = ny;
+                z1 = nz;
+                certified = true;
+            }
+        }
+    }
+    if (!certified) {
+        abs_pos(M, s.t, s.x, s.y, s.z, x0abs);
+        if (!kExact) {
+            s.x = x1;
+            s.y = y1;
+            s.z = z1;
+            return kDeferNow;
+        }
+        double p1[3];
+        abs_pos(M, s.t, x1, y1, z1, p1);
+        Resolved r = resolve(M, x0abs, p1, s.t, (s.f & kStrict) != 0);
+        exh += r.exhaustive;
+        err |= r.err;
+        if (r.hit >= 0) {
+            hit_tri = r.hit;
+            s.x = x1;
+            s.y = y1;
+            s.z = z1;
+            return kSettledNow;
+        }
+        R nx, ny, nz;
+        to_frame(M, r.tet, p1, nx, ny, nz);
+        int strict = r.strict;
+        if (!std::is_same<R, double>::value && r.tet >= 0) {
+            // the stored (rounded) position decides the fast path next step
+            double q[3];
+            abs_pos(M, r.tet, nx, ny, nz, q);
+            strict = tet_contains(M, r.tet, q) == 2;
+        }
+        s.t = r.tet;
+        s.f = strict ? (s.f | kStrict) : (s.f & ~kStrict);
+        x1 = nx;
+        y1 = ny;
+        z1 = nz;
+        if (s.t >= 0) bary_at(M, s.t, x1, y1, z1, lam);
+    }
+    s.x = x1;
+    s.y = y1;
+    s.z = z1;
+    if (S.charge && s.t >= 0 && k.charge != 0.0) sink.add4(M.tnode[s.t], k.charge, lam);
+    return kDone;
+}
+
+// ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
 __global__ void prologue_kernel(double* __restrict__ a, double* __restrict__ b, int64_t n_nodes)
@@ -303,8 +670,11 @@ __global__ void prologue_kernel(double* __restrict__ a, double* __restrict__ b, 
     }
 }
 
+// Streaming kernel: one thread per particle, S.nsteps steps with the state and
+// the owning tet's data in registers.  A particle whose relocation cannot be
+// certified stops and goes to the work list with the step it stopped at.
 template <typename R>
-__global__ void __launch_bounds__(256) step_main(DevMesh M, DevParticles<R> P, StepParams S,
+__global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P, StepParams S,
                                                  const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
                                                  WorkItem* __restrict__ wl, unsigned long long* __restrict__ cnt,
                                                  unsigned long long* __restrict__ tri_count)
@@ -313,7 +683,7 @@ __global__ void __launch_bounds__(256) step_main(DevMesh M, DevParticles<R> P, S
     __shared__ int32_t hk[kHashSlots];
     __shared__ double hv[kHashSlots];
     {
-        // clear the next step's charge buffer and work-list counter
+        // clear the next launch's charge buffer and work-list counter
         int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
         int64_t gs = (int64_t)gridDim.x * blockDim.x;
         if (S.zero_charge)
@@ -330,120 +700,60 @@ __global__ void __launch_bounds__(256) step_main(DevMesh M, DevParticles<R> P, S
         }
         __syncthreads();
     }
+    LdsSink sink{hk, hv, node_charge};
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool proc = false;
+    int processed = 0;
     bool deferred = false;
-    double xd = 0.0, yd = 0.0, zd = 0.0;
-    int32_t t0 = -1;
+    int defer_k = 0;
+    double x0abs[3] = {0.0, 0.0, 0.0};
+    int32_t t_defer = -1;
     if (i < n_active) {
-        int32_t f = P.flags[i];
-        if (!(f & kSettled)) {
-            const SpeciesK& k = sp[(f >> kSpeciesShift) & 0xff];
-            R x = P.x[i], y = P.y[i], z = P.z[i];
-            R vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
-            t0 = P.tet[i];
-            int32_t t1 = t0;  // owner after the step
-            // 2. |v| < 10 m/s and 3. outside the expanded bbox freeze the particle
-            //    for this step (ParticleDynamicsProcessor.cpp:170-176)
-            R vmod = m_sqrt<R>(vx * vx + vy * vy + vz * vz);
-            bool frozen = vmod < R(10);
-            xd = (double)x;
-            yd = (double)y;
-            zd = (double)z;
-            frozen = frozen || !(xd >= S.lo[0] && xd <= S.hi[0] && yd >= S.lo[1] && yd <= S.hi[1] &&
-                                 zd >= S.lo[2] && zd <= S.hi[2]);
-            bool have_lam = false;
-            bool settled_now = false;
-            double lam[4];
-            if (!frozen) {
-                proc = true;
-                R dt = (R)S.dt;
-                // 4. push with the owning tet's field (ParticleDynamicsProcessor.cpp:179-180)
-                if (S.em_push && t0 >= 0) {
-                    const double* E = M.E + 3 * (int64_t)t0;
-                    push<R>((R)E[0], (R)E[1], (R)E[2], (R)k.charge, (R)k.mass, dt, vx, vy, vz);
+        PState<R> s;
+        s.f = P.flags[i];
+        if (!(s.f & kSettled)) {
+            const SpeciesK& k = sp[(s.f >> kSpeciesShift) & 0xff];
+            s.x = P.x[i];
+            s.y = P.y[i];
+            s.z = P.z[i];
+            s.vx = P.vx[i];
+            s.vy = P.vy[i];
+            s.vz = P.vz[i];
+            s.t = P.tet[i];
+            const int32_t t_in = s.t, f_in = s.f;
+            const uint32_t pidx = S.model != 0 ? P.pidx[i] : 0u;
+            int32_t hit_tri = -1;
+            int exh = 0, err = 0;
+            bool moved = false;
+            for (int ks = 0; ks < S.nsteps; ++ks) {
+                int t_before = s.t;
+                int r = step_one<R, false>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink, hit_tri,
+                                           x0abs, exh, err);
+                if (r == kFrozenNow) break;
+                ++processed;
+                moved = true;
+                if (r == kSettledNow) {
+                    s.f = (s.f | kSettled) & ~kStrict;
+                    P.hit[i] = hit_tri;
+                    atomicAdd(&tri_count[hit_tri], 1ull);
+                    atomicAdd(&cnt[kCntSettled], 1ull);
+                    break;
                 }
-                // 6. move (Particle.cpp:157-165)
-                R x1 = x + vx * dt, y1 = y + vy * dt, z1 = z + vz * dt;
-                if (!(x1 == x && y1 == y && z1 == z)) {
-                    // 7. gas collision (ParticlePhysicsUpdater.cpp:20-24)
-                    if (S.model != 0) {
-                        double u1, u1b;
-                        variates(S.seed, S.run, P.pidx[i], S.step, 0u, u1, u1b);
-                        CollideK<R> ck{(R)k.sigma_hs, (R)k.vhs_c, (R)k.two_kt, (R)k.m_r, (R)k.vhs_exp,
-                                       (R)k.mass_cp, (R)k.mass_ct, (R)k.inv_alpha};
-                        R vm = m_sqrt<R>(vx * vx + vy * vy + vz * vz);
-                        R pc = collision_probability<R>(S.model, ck, (R)S.conc, dt, vm);
-                        if ((R)u1 < pc) {
-                            double u2, u3;
-                            variates(S.seed, S.run, P.pidx[i], S.step, 1u, u2, u3);
-                            scatter_velocity<R>(S.model, ck, (R)u2, (R)u3, vx, vy, vz);
-                        }
-                    }
-                    // 8. relocation and surface hit on the segment x -> x1.
-                    //    Certified cases: x1 strictly inside the owner (no face met), or
-                    //    x1 strictly outside exactly one face f and strictly inside the
-                    //    other three, so the segment leaves through the open face f.
-                    //    Everything else goes to the exact walk of step_resolve.
-                    if (t0 >= 0 && (f & kStrict)) {
-                        const double tau = M.tau;
-                        bary64(M.geo + 12 * (int64_t)t0, (double)x1, (double)y1, (double)z1, lam);
-                        int npos = 0, nneg = 0, fneg = 0;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            npos += lam[q] > tau;
-                            if (lam[q] < -tau) {
-                                ++nneg;
-                                fneg = q;
-                            }
-                        }
-                        if (npos == 4) {
-                            have_lam = true;
-                        } else if (npos == 3 && nneg == 1) {
-                            int32_t code = nth(M.tnbr[t0], fneg);
-                            if (code <= -2) {
-                                settled_now = true;  // crosses the surface triangle -2-code
-                                int32_t tri = -2 - code;
-                                P.flags[i] = (f | kSettled) & ~kStrict;
-                                P.hit[i] = tri;
-                                atomicAdd(&tri_count[tri], 1ull);
-                                atomicAdd(&cnt[kCntSettled], 1ull);
-                            } else if (code == -1) {
-                                t1 = -1;  // left through a face with no triangle
-                                P.tet[i] = -1;
-                                P.flags[i] = f & ~kStrict;
-                            } else {
-                                bary64(M.geo + 12 * (int64_t)code, (double)x1, (double)y1, (double)z1, lam);
-                                double mn = fmin(fmin(lam[0], lam[1]), fmin(lam[2], lam[3]));
-                                if (mn > tau) {
-                                    t1 = code;
-                                    have_lam = true;
-                                    P.tet[i] = code;
-                                } else {
-                                    deferred = true;
-                                }
-                            }
-                        } else {
-                            deferred = true;
-                        }
-                    } else {
-                        deferred = true;
-                    }
-                    P.x[i] = x1;
-                    P.y[i] = y1;
-                    P.z[i] = z1;
-                    x = x1;
-                    y = y1;
-                    z = z1;
+                if (r == kDeferNow) {
+                    deferred = true;
+                    defer_k = ks;
+                    t_defer = t_before;
+                    break;
                 }
-                P.vx[i] = vx;
-                P.vy[i] = vy;
-                P.vz[i] = vz;
             }
-            // node charge of the end-of-step position (the next gather)
-            if (S.charge && !deferred && !settled_now && t1 >= 0 && k.charge != 0.0) {
-                if (!have_lam) bary64(M.geo + 12 * (int64_t)t1, (double)x, (double)y, (double)z, lam);
-                lds_scatter4(hk, hv, node_charge, M.tnode[t1], k.charge, lam);
+            if (moved) {
+                P.x[i] = s.x;
+                P.y[i] = s.y;
+                P.z[i] = s.z;
+                P.vx[i] = s.vx;
+                P.vy[i] = s.vy;
+                P.vz[i] = s.vz;
+                if (s.t != t_in) P.tet[i] = s.t;
+                if (s.f != f_in) P.flags[i] = s.f;
             }
         }
     }
@@ -458,18 +768,22 @@ __global__ void __launch_bounds__(256) step_main(DevMesh M, DevParticles<R> P, S
         base = __shfl(base, leader);
         if (deferred) {
             WorkItem it;
-            it.x0 = xd;
-            it.y0 = yd;
-            it.z0 = zd;
+            it.x0 = x0abs[0];
+            it.y0 = x0abs[1];
+            it.z0 = x0abs[2];
             it.i = (int32_t)i;
-            it.t0 = t0;
+            it.t0 = t_defer;
+            it.k = defer_k;
+            it.pad = 0;
             wl[base + (unsigned)__popcll(dm & ((1ull << lane) - 1ull))] = it;
         }
     }
-    // processed count: LDS per block, one atomic per block spread over slots
+    // processed count: wavefront sum, LDS per block, one atomic per block spread over slots
     __syncthreads();
-    unsigned long long pm = __ballot(proc);
-    if ((threadIdx.x & 63) == 0 && pm) atomicAdd(&blk_processed, (unsigned int)__popcll(pm));
+    int wsum = processed;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off);
+    if ((threadIdx.x & 63) == 0 && wsum) atomicAdd(&blk_processed, (unsigned int)wsum);
     __syncthreads();
     if (threadIdx.x == 0 && blk_processed)
         atomicAdd(&cnt[kCntProcSlot0 + (blockIdx.x & (kProcSlots - 1))], (unsigned long long)blk_processed);
@@ -481,54 +795,122 @@ __global__ void __launch_bounds__(256) step_main(DevMesh M, DevParticles<R> P, S
     }
 }
 
+// Straggler kernel: finishes the deferred relocation exactly, then runs the
+// particle's remaining steps of the launch with exact relocation inline.
 template <typename R>
-__global__ void __launch_bounds__(256) step_resolve(DevMesh M, DevParticles<R> P, StepParams S,
-                                                    const SpeciesK* __restrict__ sp,
-                                                    double* __restrict__ node_charge, const WorkItem* __restrict__ wl,
-                                                    unsigned long long* __restrict__ cnt,
-                                                    unsigned long long* __restrict__ tri_count)
+__global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P, StepParams S,
+                                                   const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
+                                                   const WorkItem* __restrict__ wl,
+                                                   unsigned long long* __restrict__ cnt,
+                                                   unsigned long long* __restrict__ tri_count)
 {
     const uint64_t nw = cnt[kCntWork + 2 * S.work_slot];
+    GlobalSink sink{node_charge};
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nw;
          w += (uint64_t)gridDim.x * blockDim.x) {
         WorkItem it = wl[w];
         int64_t i = it.i;
-        double x0[3] = {it.x0, it.y0, it.z0};
-        double x1[3] = {(double)P.x[i], (double)P.y[i], (double)P.z[i]};
-        int32_t f = P.flags[i];
-        Resolved r = resolve(M, x0, x1, it.t0, (f & kStrict) != 0);
-        if (r.err) atomicOr(&cnt[kCntError], 1ull);
-        if (r.exhaustive) atomicAdd(&cnt[kCntExhaustive], 1ull);
+        PState<R> s;
+        s.f = P.flags[i];
+        s.t = it.t0;
+        s.x = P.x[i];
+        s.y = P.y[i];
+        s.z = P.z[i];
+        s.vx = P.vx[i];
+        s.vy = P.vy[i];
+        s.vz = P.vz[i];
+        const SpeciesK& k = sp[(s.f >> kSpeciesShift) & 0xff];
+        int exh = 0, err = 0;
+        // finish step it.k: exact relocation of x0 -> x1
+        double x0[3] = {it.x0, it.y0, it.z0}, x1[3];
+        abs_pos(M, s.t, s.x, s.y, s.z, x1);
+        Resolved r = resolve(M, x0, x1, s.t, (s.f & kStrict) != 0);
+        exh += r.exhaustive;
+        err |= r.err;
+        bool settled = false;
         if (r.hit >= 0) {
-            // ParticleSettler::settle (ParticleSettler.cpp:7-26)
-            P.flags[i] = (f | kSettled) & ~kStrict;
+            s.f = (s.f | kSettled) & ~kStrict;
             P.hit[i] = r.hit;
             atomicAdd(&tri_count[r.hit], 1ull);
             atomicAdd(&cnt[kCntSettled], 1ull);
+            settled = true;
         } else {
-            P.tet[i] = r.tet;
-            P.flags[i] = r.strict ? (f | kStrict) : (f & ~kStrict);
-            const SpeciesK& k = sp[(f >> kSpeciesShift) & 0xff];
-            if (S.charge && r.tet >= 0 && k.charge != 0.0) {
+            to_frame(M, r.tet, x1, s.x, s.y, s.z);
+            int strict = r.strict;
+            if (!std::is_same<R, double>::value && r.tet >= 0) {
+                double q[3];
+                abs_pos(M, r.tet, s.x, s.y, s.z, q);
+                strict = tet_contains(M, r.tet, q) == 2;
+            }
+            s.t = r.tet;
+            s.f = strict ? (s.f | kStrict) : (s.f & ~kStrict);
+            if (S.charge && s.t >= 0 && k.charge != 0.0) {
                 double lam[4];
-                bary64(M.geo + 12 * (int64_t)r.tet, x1[0], x1[1], x1[2], lam);
-                scatter4(node_charge, M.tnode[r.tet], k.charge, lam);
+                bary_at(M, s.t, s.x, s.y, s.z, lam);
+                sink.add4(M.tnode[s.t], k.charge, lam);
             }
         }
+        // remaining steps of this launch
+        if (!settled) {
+            const uint32_t pidx = S.model != 0 ? P.pidx[i] : 0u;
+            unsigned long long extra = 0;
+            for (int ks = it.k + 1; ks < S.nsteps; ++ks) {
+                int32_t hit_tri = -1;
+                double x0a[3];
+                int rr = step_one<R, true>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink, hit_tri, x0a,
+                                           exh, err);
+                if (rr == kFrozenNow) break;
+                ++extra;
+                if (rr == kSettledNow) {
+                    s.f = (s.f | kSettled) & ~kStrict;
+                    P.hit[i] = hit_tri;
+                    atomicAdd(&tri_count[hit_tri], 1ull);
+                    atomicAdd(&cnt[kCntSettled], 1ull);
+                    break;
+                }
+            }
+            if (extra) atomicAdd(&cnt[kCntProcSlot0 + (w & (kProcSlots - 1))], extra);
+        }
+        P.x[i] = s.x;
+        P.y[i] = s.y;
+        P.z[i] = s.z;
+        P.vx[i] = s.vx;
+        P.vy[i] = s.vy;
+        P.vz[i] = s.vz;
+        P.tet[i] = s.t;
+        P.flags[i] = s.f;
+        if (err) atomicOr(&cnt[kCntError], 1ull);
+        if (exh) atomicAdd(&cnt[kCntExhaustive], (unsigned long long)exh);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt[kCntDeferred], (unsigned long long)nw);
 }
 
-template <typename R>
-__global__ void locate_kernel(DevMesh M, const R* __restrict__ x, const R* __restrict__ y, const R* __restrict__ z,
-                              int64_t n, int32_t* __restrict__ tet, int32_t* __restrict__ flags,
-                              unsigned long long* __restrict__ cnt)
+// Owners of absolute positions (min-id closed containment); with `lx` the
+// positions are also written in the frame of their owner (mixed layout).
+__global__ void locate_kernel(DevMesh M, const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, int64_t n, int32_t* __restrict__ tet,
+                              int32_t* __restrict__ flags, float* __restrict__ lx, float* __restrict__ ly,
+                              float* __restrict__ lz, unsigned long long* __restrict__ cnt)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double p[3] = {(double)x[i], (double)y[i], (double)z[i]};
+    double p[3] = {x[i], y[i], z[i]};
     int strict = 0, err = 0;
     int32_t t = locate_grid(M, p, &strict, &err);
+    if (lx) {
+        float a, b, c;
+        to_frame(M, t, p, a, b, c);
+        lx[i] = a;
+        ly[i] = b;
+        lz[i] = c;
+        if (t >= 0) {
+            double q[3];
+            abs_pos(M, t, a, b, c, q);
+            int r = tet_contains(M, t, q);
+            if (r < 0) err = 1;
+            strict = r == 2;
+        }
+    }
     if (err && cnt) atomicOr(&cnt[kCntError], 1ull);
     tet[i] = t;
     if (flags) flags[i] = strict ? (flags[i] | kStrict) : (flags[i] & ~kStrict);
@@ -546,7 +928,7 @@ __global__ void gather_kernel(DevMesh M, DevParticles<R> P, const SpeciesK* __re
     double q = sp[(f >> kSpeciesShift) & 0xff].charge;
     if (q == 0.0) return;
     double lam[4];
-    bary64(M.geo + 12 * (int64_t)t, (double)P.x[i], (double)P.y[i], (double)P.z[i], lam);
+    bary_at(M, t, P.x[i], P.y[i], P.z[i], lam);
     scatter4(node_charge, M.tnode[t], q, lam);
 }
 
@@ -590,6 +972,20 @@ __global__ void orient_batch_kernel(const double* __restrict__ pts, int64_t n, i
     out[i] = orient3d(p, p + 3, p + 6, p + 9);
 }
 
+// absolute FP64 positions of the mixed layout, for read-back
+__global__ void abs_positions_kernel(DevMesh M, const float* __restrict__ lx, const float* __restrict__ ly,
+                                     const float* __restrict__ lz, const int32_t* __restrict__ tet, int64_t n,
+                                     double* __restrict__ x, double* __restrict__ y, double* __restrict__ z)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double p[3];
+    abs_pos(M, tet[i], lx[i], ly[i], lz[i], p);
+    x[i] = p[0];
+    y[i] = p[1];
+    z[i] = p[2];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -608,15 +1004,14 @@ void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, cons
 {
     if (P.n > 0) step_main<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
-    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 256), 64u));
-    step_resolve<R><<<nb, 256, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
+    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 256u));
+    step_resolve<R><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
 }
 
-template <typename R>
-void launch_locate(hipStream_t s, const DevMesh& M, const R* x, const R* y, const R* z, int64_t n, int32_t* tet,
-                   int32_t* flags, unsigned long long* cnt)
+void launch_locate(hipStream_t s, const DevMesh& M, const double* x, const double* y, const double* z, int64_t n,
+                   int32_t* tet, int32_t* flags, float* lx, float* ly, float* lz, unsigned long long* cnt)
 {
-    if (n > 0) locate_kernel<R><<<blocks_for(n, 128), 128, 0, s>>>(M, x, y, z, n, tet, flags, cnt);
+    if (n > 0) locate_kernel<<<blocks_for(n, 128), 128, 0, s>>>(M, x, y, z, n, tet, flags, lx, ly, lz, cnt);
 }
 
 template <typename R>
@@ -640,12 +1035,21 @@ void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* o
     if (n > 0) orient_batch_kernel<<<blocks_for(n, 64), 64, 0, s>>>(pts, n, out);
 }
 
+void launch_abs_positions(hipStream_t s, const DevMesh& M, const float* lx, const float* ly, const float* lz,
+                          const int32_t* tet, int64_t n, double* x, double* y, double* z)
+{
+    if (n > 0) abs_positions_kernel<<<blocks_for(n, 256), 256, 0, s>>>(M, lx, ly, lz, tet, n, x, y, z);
+}
+
 template void launch_step<double>(hipStream_t, const DevMesh&, const DevParticles<double>&, const StepParams&,
                                   const SpeciesK*, double*, WorkItem*, unsigned long long*, unsigned long long*,
                                   hipEvent_t);
-template void launch_locate<double>(hipStream_t, const DevMesh&, const double*, const double*, const double*, int64_t,
-                                    int32_t*, int32_t*, unsigned long long*);
+template void launch_step<float>(hipStream_t, const DevMesh&, const DevParticles<float>&, const StepParams&,
+                                 const SpeciesK*, double*, WorkItem*, unsigned long long*, unsigned long long*,
+                                 hipEvent_t);
 template void launch_gather<double>(hipStream_t, const DevMesh&, const DevParticles<double>&, const SpeciesK*,
                                     double*);
+template void launch_gather<float>(hipStream_t, const DevMesh&, const DevParticles<float>&, const SpeciesK*,
+                                   double*);
 
 }  // namespace sg

@@ -24,7 +24,8 @@ def log(msg):
 
 
 class Pair:
-    def __init__(self, w, n_particles=None, em_push=True, charge=True, E=None):
+    def __init__(self, w, n_particles=None, em_push=True, charge=True, E=None, steps_per_launch=1,
+                 precision="fp64", sort_interval=20):
         self.w = w
         n = len(w.pos) if n_particles is None else n_particles
         self.n = n
@@ -41,16 +42,21 @@ class Pair:
                          em_push=em_push, E=self.E)
         self.charge = charge
         self.qo = np.zeros(w.mesh.n_nodes)
+        self.K = steps_per_launch
+        self.precision = precision
         self.eng = ParticleInCellStep(w.mesh, w.species, g, w.dt, w.seed, em_push=em_push,
-                                      charge_every_step=charge)
+                                      charge_every_step=charge, steps_per_launch=steps_per_launch,
+                                      precision=precision, sort_interval=sort_interval)
         self.eng.load_particles(pos, vel, sp)
         self.eng.set_tet_field(self.E)
         self.step = 0
         self.processed_o = 0
 
     def advance(self, k):
-        for _ in range(k):
-            if self.charge:
+        """k steps; the node charge of a launch of K fused steps is the sum of
+        the per-step charges of those steps."""
+        for j in range(k):
+            if self.charge and j % self.K == 0:
                 self.qo[:] = 0.0
             self.processed_o += self.sim.step(self.st, self.step, self.qo if self.charge else None,
                                               nthreads=8)

@@ -114,3 +114,69 @@ def test_degenerate_starts_on_vertices_edges_faces():
         pr.advance(20)
         pr.compare(f"degenerate starts step {pr.step}")
     pr.close()
+
+
+@pytest.mark.parametrize("K", [4, 25])
+def test_fused_launch_parity(K):
+    """K steps fused in one launch (state in registers, stragglers finished by
+    the exact kernel): same bit-exact bar, charge summed over each launch."""
+    w = workloads.large_box(n_particles=20_000, cells=10, model="VSS")
+    w.gas.concentration *= 100.0
+    w.dt = 2e-8
+    pr = Pair(w, steps_per_launch=K)
+    for k in range(2):
+        pr.advance(100)
+        r = pr.compare(f"fused K={K} step {pr.step}")
+    assert r["owner_changed"] > 500 and r["settled"] > 50
+    pr.close()
+
+
+def test_fused_small_box_jet():
+    w = workloads.small_box(n_particles=3000, charge_units=2.0)
+    pr = Pair(w, steps_per_launch=10)
+    for k in range(3):
+        pr.advance(100)
+        pr.compare(f"fused small_box step {pr.step}")
+    pr.close()
+
+
+@pytest.mark.parametrize("K", [1, 10])
+def test_mixed_precision_against_fp64_oracle(K):
+    """FP32 push/collision/state with FP64 containment near faces and FP64
+    charge accumulation, against the FP64 oracle.  Bar, stated per result, for
+    particles whose owner and status agree with the oracle:
+      x: every particle within 1e-5 of |x|;
+      v: 99.5% of particles within 1e-5 of |v|, and every particle within
+         1e-5 x (3 x thermal speed) absolute -- a collision that nearly cancels
+         the lab-frame velocity (|v'| ~ 1 m/s from two ~1,250 m/s terms) keeps
+         the FP32 state's absolute error (~1e-7 of the pre-collision speed);
+    owner or status disagreement for fewer than 0.5% of particles; node charge
+    L1 error within 1e-4 of the total."""
+    w = workloads.large_box(n_particles=20_000, cells=10)
+    w.gas.concentration *= 20.0
+    w.dt = 1e-8
+    pr = Pair(w, steps_per_launch=K, precision="mixed")
+    pr.advance(200)
+    g = pr.eng.particles()
+    st = pr.st
+    same = (g["tet"] == st.tet) & (g["status"] == st.status)
+    frac_diff = 1.0 - same.mean()
+    xo = np.stack([st.x, st.y, st.z], 1)[same]
+    xg = np.stack([g["x"], g["y"], g["z"]], 1)[same]
+    vo = np.stack([st.vx, st.vy, st.vz], 1)[same]
+    vg = np.stack([g["vx"], g["vy"], g["vz"]], 1)[same]
+    ex = (np.abs(xo - xg).max(1) / np.linalg.norm(xo, axis=1)).max()
+    dv = np.abs(vo - vg).max(1)
+    evs = dv / np.linalg.norm(vo, axis=1)
+    ev = float(np.quantile(evs, 0.995))
+    v_th = np.sqrt(1.0 * 1.60217733000001e-19 / w.species[0].mass)
+    ev_abs = float(dv.max() / (3.0 * v_th))
+    qg = pr.eng.node_charge()
+    eq = np.abs(qg - pr.qo).sum() / np.abs(pr.qo).sum()
+    log(f"mixed K={K}: owner/status differ {frac_diff:.2e}, max rel x {ex:.2e}, v q99.5 {ev:.2e} "
+        f"max {evs.max():.2e}, v abs/(3 v_th) {ev_abs:.2e}, charge L1 {eq:.2e}, "
+        f"settled gpu {int((g['status'] == 1).sum())} oracle {int((st.status == 1).sum())}")
+    assert frac_diff < 5e-3
+    assert ex < 1e-5 and ev < 1e-5 and ev_abs < 1e-5
+    assert eq < 1e-4
+    pr.close()

@@ -11,7 +11,8 @@ A step is one pass of the hot path over all particles: push, move, collision,
 relocation, surface hit/settle, charge scatter.  The timed region holds exactly
 --steps steps with the particles resident in HBM, bracketed by device
 synchronisation.  value = particle-steps processed (not settled, not frozen)
-per second.
+per second, in the headline mode (FP64, bit-exact with the CPU oracle); the
+other modes are reported under "modes".
 
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -29,8 +30,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-# algorithmic bytes per particle-step: 2*(6s+8) + (3 s_E + 12 s_G + 32) + 32
+# algorithmic bytes per particle-step of the streaming step:
+#   2*(6s + 8) particle state + (3 s_E + 12 s_G + 16 + 16) tet record + 4*8 charge
 BYTES_PER_PS = {"fp64": 296, "mixed": 188}
+
+MODES = {
+    # name: (precision, steps fused per launch, sort interval)
+    "fp64_fused": ("fp64", 100, 200),
+    "fp64_per_step": ("fp64", 1, 50),
+    "mixed_fused": ("mixed", 100, 200),
+    "mixed_per_step": ("mixed", 1, 50),
+}
+HEADLINE = "fp64_fused"
 
 
 def log(msg):
@@ -50,9 +61,10 @@ def cpu_threads() -> int:
 
 def gpu_run(w, mode: str, steps: int, warmup: int):
     from start_amd.engine import ParticleInCellStep
+    prec, K, sort = MODES[mode]
     t_setup = time.time()
-    eng = ParticleInCellStep(w.mesh, w.species, w.gas, w.dt, w.seed, precision=mode,
-                             charge_every_step=True)
+    eng = ParticleInCellStep(w.mesh, w.species, w.gas, w.dt, w.seed, precision=prec,
+                             charge_every_step=True, steps_per_launch=K, sort_interval=sort)
     eng.load_particles(w.pos, w.vel, w.sp_idx)
     eng.set_node_potential(w.phi)
     eng.synchronize()
@@ -60,23 +72,43 @@ def gpu_run(w, mode: str, steps: int, warmup: int):
     eng.process(warmup)
     eng.synchronize()
     s0 = eng.stats()
-    eng.timing(True)
-    eng.synchronize()
+    # throughput: no event records in the timed region
     t0 = time.perf_counter()
     eng.process(steps)
     eng.synchronize()
     t1 = time.perf_counter()
-    kern_ms, launches = eng.timing_read()
     s1 = eng.stats()
+    # kernel duration of step_main with HIP events on the library's stream
+    tsteps = min(steps, max(K, 200 if K == 1 else 2 * K))
+    eng.timing(True)
+    eng.process(tsteps)
+    eng.synchronize()
+    kern_ms, launches = eng.timing_read()
+    s2 = eng.stats()
     eng.close()
     processed = s1["processed"] - s0["processed"]
     wall = t1 - t0
     return {
+        "precision": prec, "steps_per_launch": K, "sort_interval": sort,
         "processed": processed, "wall_s": wall, "ps_per_s": processed / wall,
         "ms_per_step": 1e3 * wall / max(steps, 1),
-        "kernel_ms_avg": kern_ms / max(launches, 1), "launches": launches,
+        "kernel_ms_per_launch": kern_ms / max(launches, 1), "launches_timed": launches,
+        "processed_per_launch": (s2["processed"] - s1["processed"]) / max(launches, 1),
         "deferred": s1["deferred"] - s0["deferred"], "settled": s1["settled"],
     }
+
+
+def roofline(r, mode):
+    prec = MODES[mode][0]
+    B = BYTES_PER_PS[prec]
+    dur = r["kernel_ms_per_launch"] * 1e-3
+    achieved = B * r["processed_per_launch"] / dur / 1e9 if dur > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(mode),
+            "kernel": f"step_main<{'double' if prec == 'fp64' else 'float'}>",
+            "bytes_per_particle_step": B, "steps_per_launch": MODES[mode][1],
+            "particle_steps_per_launch": round(r["processed_per_launch"]),
+            "launch_ms": round(r["kernel_ms_per_launch"], 4)}
 
 
 def cpu_baseline(w, budget_s: float = 12.0, max_steps: int = 200):
@@ -93,7 +125,6 @@ def cpu_baseline(w, budget_s: float = 12.0, max_steps: int = 200):
     sim = O.Sim(om, ks, O.MODEL[g.model], g.concentration, w.dt, w.seed, E=E)
     q = np.zeros(w.mesh.n_nodes)
     nth = cpu_threads()
-    q[:] = 0
     sim.step(st, 0, q, nthreads=nth)  # warm-up step
     done, processed, t0 = 0, 0, time.perf_counter()
     while done < max_steps and time.perf_counter() - t0 < budget_s:
@@ -101,13 +132,13 @@ def cpu_baseline(w, budget_s: float = 12.0, max_steps: int = 200):
         processed += sim.step(st, 1 + done, q, nthreads=nth)
         done += 1
     el = time.perf_counter() - t0
-    return {"value": processed / el, "unit": "particle-steps/s", "cores": nth, "kind": "port",
+    return {"value": round(processed / el, 1), "unit": "particle-steps/s", "cores": nth, "kind": "port",
             "sample": f"oracle on the same large_box inputs, steps 1..{done} after 1 warm-up "
                       f"step ({processed} particle-steps in {el:.1f} s, OpenMP {nth} threads)"}
 
 
 def load_traffic(mode: str):
-    """HBM bytes per launch of step_main from the committed rocprofv3 summary."""
+    """HBM bytes per launch of step_main, from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -122,11 +153,24 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--modes", default=",".join(MODES), help="comma-separated subset of " + ",".join(MODES))
+    ap.add_argument("--mode-steps", type=int, default=400, help="timed steps of the non-headline modes")
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES, 5 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the particle step "
               "is measured per GPU and its multi-GPU sharding is described, not run.", file=sys.stderr)
@@ -138,15 +182,14 @@ def main():
     log(f"[bench] workload built in {time.time() - t:.1f}s: {w.mesh.n_tets} tets, "
         f"{w.mesh.n_nodes} nodes, {w.mesh.n_tris} triangles, {len(w.pos)} particles")
 
-    r = gpu_run(w, args.mode, args.steps, args.warmup)
-    log(f"[bench] gpu {args.mode}: {r}")
-    per_launch = r["processed"] / max(args.steps, 1)
-    B = BYTES_PER_PS[args.mode]
-    achieved = B * per_launch / (r["kernel_ms_avg"] * 1e-3) / 1e9 if r["kernel_ms_avg"] > 0 else 0.0
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(args.mode),
-            "kernel": "step_main", "bytes_per_particle_step": B,
-            "kernel_ms_avg": round(r["kernel_ms_avg"], 5)}
+    modes = [m for m in args.modes.split(",") if m]
+    results = {}
+    for m in modes:
+        steps = args.steps if m == HEADLINE else args.mode_steps
+        results[m] = gpu_run(w, m, steps, args.warmup)
+        log(f"[bench] {m}: {results[m]}")
+    head = HEADLINE if HEADLINE in results else modes[0]
+    r = results[head]
 
     cpu = None
     if not args.no_cpu:
@@ -159,21 +202,27 @@ def main():
         "value": round(r["ps_per_s"], 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": args.steps if head == HEADLINE else args.mode_steps,
         "warmup": args.warmup,
         "ms_per_step": round(r["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.mode == "fp64" else "f32+f64",
+        "dtype": "f64" if MODES[head][0] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "large_box: 750,000 tets, 1,000,000 Ar+ (1 eV Maxwellian), VHS Ar 1 Pa "
                                "300 K, E from 0/500 V, charge scatter every step, dt 1e-9 s",
-                   "mode": args.mode, "particles": len(w.pos), "tets": w.mesh.n_tets,
-                   "particle_steps_timed": r["processed"], "parallelism": "1 GPU"},
-        "roofline": roof,
+                   "mode": head, "steps_per_launch": MODES[head][1], "particles": len(w.pos),
+                   "tets": w.mesh.n_tets, "particle_steps_timed": r["processed"], "parallelism": "1 GPU"},
+        "roofline": roofline(r, head),
         "cpu_baseline": cpu,
+        "modes": {m: {"value": round(v["ps_per_s"], 1), "ms_per_step": round(v["ms_per_step"], 5),
+                      "steps": args.steps if m == HEADLINE else args.mode_steps,
+                      "precision": v["precision"], "steps_per_launch": v["steps_per_launch"],
+                      "roofline": roofline(v, m)} for m, v in results.items()},
     }
+    if cpu:
+        out["speedup_vs_cpu_baseline"] = round(r["ps_per_s"] / cpu["value"], 1)
     print(json.dumps(out), flush=True)
 
 

@@ -463,16 +463,45 @@ struct GlobalSink {
     double* nc;
     __device__ void add4(const int4& nv, double q, const double lam[4]) { scatter4(nc, nv, q, lam); }
 };
+// Per-particle accumulation across the steps of a launch: a particle that
+// stays in one tet adds q*lambda into four registers, and its sums go to the
+// inner sink only when it changes tet or the launch ends.
+template <typename Inner>
+struct RegSink {
+    Inner inner;
+    int32_t cur;
+    double a0, a1, a2, a3;
+    __device__ void flush(const DevMesh& M)
+    {
+        if (cur >= 0) {
+            double v[4] = {a0, a1, a2, a3};
+            inner.add4(M.tnode[cur], 1.0, v);
+        }
+        cur = -1;
+        a0 = a1 = a2 = a3 = 0.0;
+    }
+    __device__ void add(const DevMesh& M, int32_t t, double q, const double lam[4])
+    {
+        if (t != cur) {
+            flush(M);
+            cur = t;
+        }
+        a0 += q * lam[0];
+        a1 += q * lam[1];
+        a2 += q * lam[2];
+        a3 += q * lam[3];
+    }
+};
 
 // Collision decision u1 < P (HSModel.cpp:15-19, VHSModel.cpp:60-70).  For
 // VHS/VSS the FP64 probability needs pow() every step; an FP32 estimate with a
 // relative error far below 1e-4 decides all draws outside a +-1e-4 band
 // around it, and only draws inside the band evaluate the FP64 expression, so
 // the FP64 decision is reproduced exactly at FP32 cost.
-__device__ __noinline__ bool collides_exact64(int model, const SpeciesK& k, double conc, double dt, double vm,
-                                              double u1)
+__device__ __noinline__ bool collides_exact64(int model, double sigma_hs, double vhs_c, double two_kt, double m_r,
+                                              double vhs_exp, double conc, double dt, double vm, double u1)
 {
-    CollideK<double> ck{k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, k.mass_cp, k.mass_ct, k.inv_alpha};
+    CollideK<double> ck{sigma_hs, vhs_c, two_kt, m_r, vhs_exp, 0.0, 0.0, 0.0};
     return u1 < collision_probability<double>(model, ck, conc, dt, vm);
 }
 
@@ -491,19 +520,47 @@ __device__ inline bool collides(const StepParams& S, const SpeciesK& k, R vx, R 
     if (!std::is_same<R, double>::value) return (float)u1 < p32;
     double p = (double)p32;
     if (!(p > 1e-30 && p < 1e30))  // outside the range the FP32 bound covers
-        return collides_exact64(S.model, k, S.conc, S.dt, (double)vm, u1);
+        return collides_exact64(S.model, k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, S.conc, S.dt, (double)vm, u1);
     if (u1 < p * (1.0 - 1e-4)) return true;
     if (u1 >= p * (1.0 + 1e-4)) return false;
-    return collides_exact64(S.model, k, S.conc, S.dt, (double)vm, u1);
+    return collides_exact64(S.model, k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, S.conc, S.dt, (double)vm, u1);
 }
 
-// post-collision velocity (rare: out of line to keep the streaming kernel lean)
+// post-collision velocity (rare: out of line to keep the streaming kernel
+// lean; arguments and result by value so nothing of the caller escapes to
+// memory)
 template <typename R>
-__device__ __noinline__ void collide_velocity(int model, const SpeciesK& k, double u2, double u3, R& vx, R& vy, R& vz)
+struct V3 {
+    R x, y, z;
+};
+template <typename R>
+__device__ __noinline__ V3<R> collide_velocity(int model, double mass_cp, double mass_ct, double inv_alpha, double u2,
+                                               double u3, R vx, R vy, R vz)
 {
-    CollideK<R> ck{(R)k.sigma_hs, (R)k.vhs_c, (R)k.two_kt, (R)k.m_r, (R)k.vhs_exp,
-                   (R)k.mass_cp, (R)k.mass_ct, (R)k.inv_alpha};
+    CollideK<R> ck{(R)0, (R)0, (R)0, (R)0, (R)0, (R)mass_cp, (R)mass_ct, (R)inv_alpha};
     scatter_velocity<R>(model, ck, (R)u2, (R)u3, vx, vy, vz);
+    return V3<R>{vx, vy, vz};
+}
+
+// Barycentric classification with a certified threshold: strictly inside,
+// strictly outside exactly one face (strictly inside the other three), or
+// uncertain.
+enum { kInside = 0, kOneFace = 1, kUncertain = 2 };
+__device__ inline int classify(const double lam[4], double tau, int& fneg)
+{
+    int npos = 0, nneg = 0;
+    fneg = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        npos += lam[q] > tau;
+        if (lam[q] < -tau) {
+            ++nneg;
+            fneg = q;
+        }
+    }
+    if (npos == 4) return kInside;
+    if (npos == 3 && nneg == 1) return kOneFace;
+    return kUncertain;
 }
 
 enum StepOutcome { kDone = 0, kSettledNow = 1, kDeferNow = 2, kFrozenNow = 3 };
@@ -515,15 +572,20 @@ struct PState {
 };
 
 // One step of one particle (ParticleDynamicsProcessor.cpp:166-199 stage order,
-// then the relocation and charge of the next gather).  kExact: resolve every
-// relocation here (the straggler kernel); otherwise uncertifiable relocations
-// return kDeferNow with the state at x1 and the start position in x0abs.
+// then the relocation and charge of the next gather).  Relocations that the
+// barycentric maps cannot certify return kDeferNow with the state at x1 in the
+// frame of the unchanged owner; the caller resolves them exactly.
 // The owning tet's data is re-read each step (L1/L2 hits while the particle
 // stays in it) to keep the register footprint of the streaming kernel small.
-template <typename R, bool kExact, typename Sink>
-__device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, const SpeciesK& k, uint32_t pidx,
-                                        uint32_t step, int remaining, PState<R>& s, Sink& sink, int32_t& hit_tri,
-                                        double* x0abs, int& exh, int& err)
+struct StepResult {
+    int outcome;
+    int32_t hit_tri;
+};
+
+template <typename R, typename Sink>
+__device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParams& S, const SpeciesK& k,
+                                               uint32_t pidx, uint32_t step, int remaining, PState<R>& s,
+                                               Sink& sink)
 {
     const double tau = tau_of<R>(M);
     // 2. |v| < 10 m/s and 3. outside the expanded bbox: frozen this step and
@@ -534,9 +596,9 @@ __device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, c
     if (frozen) {
         if (S.charge && s.t >= 0 && k.charge != 0.0) {
             bary_at(M, s.t, s.x, s.y, s.z, lam);
-            sink.add4(M.tnode[s.t], k.charge * (double)remaining, lam);
+            sink.add(M, s.t, k.charge * (double)remaining, lam);
         }
-        return kFrozenNow;
+        return StepResult{kFrozenNow, -1};
     }
     R dt = (R)S.dt;
     // 4. push with the owning tet's field (ParticleDynamicsProcessor.cpp:179-180)
@@ -550,9 +612,9 @@ __device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, c
     if (x1 == s.x && y1 == s.y && z1 == s.z) {  // degenerate segment (:188-190)
         if (S.charge && s.t >= 0 && k.charge != 0.0) {
             bary_at(M, s.t, s.x, s.y, s.z, lam);
-            sink.add4(M.tnode[s.t], k.charge, lam);
+            sink.add(M, s.t, k.charge, lam);
         }
-        return kDone;
+        return StepResult{kDone, -1};
     }
     // 7. gas collision (ParticlePhysicsUpdater.cpp:20-24)
     if (S.model != 0) {
@@ -561,7 +623,10 @@ __device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, c
         if (collides<R>(S, k, s.vx, s.vy, s.vz, u1)) {
             double u2, u3;
             variates(S.seed, S.run, pidx, step, 1u, u2, u3);
-            collide_velocity<R>(S.model, k, u2, u3, s.vx, s.vy, s.vz);
+            V3<R> nv = collide_velocity<R>(S.model, k.mass_cp, k.mass_ct, k.inv_alpha, u2, u3, s.vx, s.vy, s.vz);
+            s.vx = nv.x;
+            s.vy = nv.y;
+            s.vz = nv.z;
         }
     }
     // 8. relocation and surface hit on the segment x -> x1.  Certified cases:
@@ -571,25 +636,25 @@ __device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, c
     bool certified = false;
     if (s.t >= 0 && (s.f & kStrict)) {
         bary_at(M, s.t, x1, y1, z1, lam);
-        int npos = 0, nneg = 0, fneg = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            npos += lam[q] > tau;
-            if (lam[q] < -tau) {
-                ++nneg;
-                fneg = q;
-            }
+        int fneg = 0;
+        int cls = classify(lam, tau, fneg);
+        if (!std::is_same<R, double>::value && cls == kUncertain) {
+            // FP32 too close to a face: certify with the FP64 map at the FP64
+            // absolute position of the stored FP32 state
+            double p1[3];
+            abs_pos(M, s.t, x1, y1, z1, p1);
+            bary_at(M, s.t, p1[0], p1[1], p1[2], lam);
+            cls = classify(lam, M.tau, fneg);
         }
-        if (npos == 4) {
+        if (cls == kInside) {
             certified = true;
-        } else if (npos == 3 && nneg == 1) {
+        } else if (cls == kOneFace) {
             int32_t code = nth(M.tnbr[s.t], fneg);
             if (code <= -2) {
-                hit_tri = -2 - code;  // crosses this surface triangle first
-                s.x = x1;
+                s.x = x1;  // crosses this surface triangle first
                 s.y = y1;
                 s.z = z1;
-                return kSettledNow;
+                return StepResult{kSettledNow, -2 - code};
             }
             // to the neighbour (or out through a face with no triangle)
             double p1[3];
@@ -602,10 +667,18 @@ __device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, c
                 s.x = nx;
                 s.y = ny;
                 s.z = nz;
-                return kDone;
+                return StepResult{kDone, -1};
             }
             bary_at(M, code, nx, ny, nz, lam);
-            if (fmin(fmin(lam[0], lam[1]), fmin(lam[2], lam[3])) > tau) {
+            int f2;
+            int c2 = classify(lam, tau, f2);
+            if (!std::is_same<R, double>::value && c2 == kUncertain) {
+                double q[3];
+                abs_pos(M, code, nx, ny, nz, q);
+                bary_at(M, code, q[0], q[1], q[2], lam);
+                c2 = classify(lam, M.tau, f2);
+            }
+            if (c2 == kInside) {
                 s.t = code;
                 x1 = nx;
                 y1 = ny;
@@ -615,46 +688,16 @@ __device__ __forceinline__ int step_one(const DevMesh& M, const StepParams& S, c
         }
     }
     if (!certified) {
-        abs_pos(M, s.t, s.x, s.y, s.z, x0abs);
-        if (!kExact) {
-            s.x = x1;
-            s.y = y1;
-            s.z = z1;
-            return kDeferNow;
-        }
-        double p1[3];
-        abs_pos(M, s.t, x1, y1, z1, p1);
-        Resolved r = resolve(M, x0abs, p1, s.t, (s.f & kStrict) != 0);
-        exh += r.exhaustive;
-        err |= r.err;
-        if (r.hit >= 0) {
-            hit_tri = r.hit;
-            s.x = x1;
-            s.y = y1;
-            s.z = z1;
-            return kSettledNow;
-        }
-        R nx, ny, nz;
-        to_frame(M, r.tet, p1, nx, ny, nz);
-        int strict = r.strict;
-        if (!std::is_same<R, double>::value && r.tet >= 0) {
-            // the stored (rounded) position decides the fast path next step
-            double q[3];
-            abs_pos(M, r.tet, nx, ny, nz, q);
-            strict = tet_contains(M, r.tet, q) == 2;
-        }
-        s.t = r.tet;
-        s.f = strict ? (s.f | kStrict) : (s.f & ~kStrict);
-        x1 = nx;
-        y1 = ny;
-        z1 = nz;
-        if (s.t >= 0) bary_at(M, s.t, x1, y1, z1, lam);
+        s.x = x1;
+        s.y = y1;
+        s.z = z1;
+        return StepResult{kDeferNow, -1};
     }
     s.x = x1;
     s.y = y1;
     s.z = z1;
-    if (S.charge && s.t >= 0 && k.charge != 0.0) sink.add4(M.tnode[s.t], k.charge, lam);
-    return kDone;
+    if (S.charge && s.t >= 0 && k.charge != 0.0) sink.add(M, s.t, k.charge, lam);
+    return StepResult{kDone, -1};
 }
 
 // ---------------------------------------------------------------------------
@@ -700,7 +743,7 @@ __global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P
         }
         __syncthreads();
     }
-    LdsSink sink{hk, hv, node_charge};
+    RegSink<LdsSink> sink{LdsSink{hk, hv, node_charge}, -1, 0.0, 0.0, 0.0, 0.0};
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     int processed = 0;
     bool deferred = false;
@@ -721,27 +764,26 @@ __global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P
             s.t = P.tet[i];
             const int32_t t_in = s.t, f_in = s.f;
             const uint32_t pidx = S.model != 0 ? P.pidx[i] : 0u;
-            int32_t hit_tri = -1;
-            int exh = 0, err = 0;
             bool moved = false;
             for (int ks = 0; ks < S.nsteps; ++ks) {
-                int t_before = s.t;
-                int r = step_one<R, false>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink, hit_tri,
-                                           x0abs, exh, err);
-                if (r == kFrozenNow) break;
+                const R px = s.x, py = s.y, pz = s.z;
+                const int32_t t_before = s.t;
+                StepResult r = step_one<R>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink);
+                if (r.outcome == kFrozenNow) break;
                 ++processed;
                 moved = true;
-                if (r == kSettledNow) {
+                if (r.outcome == kSettledNow) {
                     s.f = (s.f | kSettled) & ~kStrict;
-                    P.hit[i] = hit_tri;
-                    atomicAdd(&tri_count[hit_tri], 1ull);
+                    P.hit[i] = r.hit_tri;
+                    atomicAdd(&tri_count[r.hit_tri], 1ull);
                     atomicAdd(&cnt[kCntSettled], 1ull);
                     break;
                 }
-                if (r == kDeferNow) {
+                if (r.outcome == kDeferNow) {
                     deferred = true;
                     defer_k = ks;
                     t_defer = t_before;
+                    abs_pos(M, t_before, px, py, pz, x0abs);
                     break;
                 }
             }
@@ -757,6 +799,7 @@ __global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P
             }
         }
     }
+    if (S.charge) sink.flush(M);
     // work list: one atomic per wavefront, lanes take consecutive slots
     unsigned long long dm = __ballot(deferred);
     if (dm) {
@@ -795,8 +838,43 @@ __global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P
     }
 }
 
-// Straggler kernel: finishes the deferred relocation exactly, then runs the
-// particle's remaining steps of the launch with exact relocation inline.
+// Exact relocation of a deferred step: the state holds x1 in the frame of the
+// owner at the step start; x0abs is the absolute start.  Returns true when the
+// particle settled.
+template <typename R, typename Sink>
+__device__ inline bool finish_exact(const DevMesh& M, const StepParams& S, const SpeciesK& k, const double* x0abs,
+                                    PState<R>& s, Sink& sink, int32_t& hit_tri, int& exh, int& err)
+{
+    double x1[3];
+    abs_pos(M, s.t, s.x, s.y, s.z, x1);
+    Resolved r = resolve(M, x0abs, x1, s.t, (s.f & kStrict) != 0);
+    exh += r.exhaustive;
+    err |= r.err;
+    if (r.hit >= 0) {
+        hit_tri = r.hit;
+        return true;
+    }
+    to_frame(M, r.tet, x1, s.x, s.y, s.z);
+    int strict = r.strict;
+    if (!std::is_same<R, double>::value && r.tet >= 0) {
+        // the stored (rounded) position decides the fast path next step
+        double q[3];
+        abs_pos(M, r.tet, s.x, s.y, s.z, q);
+        strict = tet_contains(M, r.tet, q) == 2;
+    }
+    s.t = r.tet;
+    s.f = strict ? (s.f | kStrict) : (s.f & ~kStrict);
+    if (S.charge && s.t >= 0 && k.charge != 0.0) {
+        double lam[4];
+        bary_at(M, s.t, s.x, s.y, s.z, lam);
+        sink.add(M, s.t, k.charge, lam);
+    }
+    return false;
+}
+
+// Straggler kernel: finishes each deferred relocation exactly, then runs the
+// particle's remaining steps of the launch with the certified fast path,
+// resolving exactly again whenever a step defers.
 template <typename R>
 __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P, StepParams S,
                                                    const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
@@ -805,9 +883,9 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
                                                    unsigned long long* __restrict__ tri_count)
 {
     const uint64_t nw = cnt[kCntWork + 2 * S.work_slot];
-    GlobalSink sink{node_charge};
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nw;
          w += (uint64_t)gridDim.x * blockDim.x) {
+        RegSink<GlobalSink> sink{GlobalSink{node_charge}, -1, 0.0, 0.0, 0.0, 0.0};
         WorkItem it = wl[w];
         int64_t i = it.i;
         PState<R> s;
@@ -820,57 +898,40 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
         s.vy = P.vy[i];
         s.vz = P.vz[i];
         const SpeciesK& k = sp[(s.f >> kSpeciesShift) & 0xff];
-        int exh = 0, err = 0;
-        // finish step it.k: exact relocation of x0 -> x1
-        double x0[3] = {it.x0, it.y0, it.z0}, x1[3];
-        abs_pos(M, s.t, s.x, s.y, s.z, x1);
-        Resolved r = resolve(M, x0, x1, s.t, (s.f & kStrict) != 0);
-        exh += r.exhaustive;
-        err |= r.err;
-        bool settled = false;
-        if (r.hit >= 0) {
-            s.f = (s.f | kSettled) & ~kStrict;
-            P.hit[i] = r.hit;
-            atomicAdd(&tri_count[r.hit], 1ull);
-            atomicAdd(&cnt[kCntSettled], 1ull);
-            settled = true;
-        } else {
-            to_frame(M, r.tet, x1, s.x, s.y, s.z);
-            int strict = r.strict;
-            if (!std::is_same<R, double>::value && r.tet >= 0) {
-                double q[3];
-                abs_pos(M, r.tet, s.x, s.y, s.z, q);
-                strict = tet_contains(M, r.tet, q) == 2;
-            }
-            s.t = r.tet;
-            s.f = strict ? (s.f | kStrict) : (s.f & ~kStrict);
-            if (S.charge && s.t >= 0 && k.charge != 0.0) {
-                double lam[4];
-                bary_at(M, s.t, s.x, s.y, s.z, lam);
-                sink.add4(M.tnode[s.t], k.charge, lam);
-            }
-        }
-        // remaining steps of this launch
+        int exh = 0, err = 0, nexact = 0;
+        int32_t hit_tri = -1;
+        double x0[3] = {it.x0, it.y0, it.z0};
+        bool settled = finish_exact(M, S, k, x0, s, sink, hit_tri, exh, err);
         if (!settled) {
             const uint32_t pidx = S.model != 0 ? P.pidx[i] : 0u;
             unsigned long long extra = 0;
             for (int ks = it.k + 1; ks < S.nsteps; ++ks) {
-                int32_t hit_tri = -1;
-                double x0a[3];
-                int rr = step_one<R, true>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink, hit_tri, x0a,
-                                           exh, err);
-                if (rr == kFrozenNow) break;
+                const R px = s.x, py = s.y, pz = s.z;
+                const int32_t t_before = s.t;
+                StepResult rr = step_one<R>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink);
+                if (rr.outcome == kFrozenNow) break;
                 ++extra;
-                if (rr == kSettledNow) {
-                    s.f = (s.f | kSettled) & ~kStrict;
-                    P.hit[i] = hit_tri;
-                    atomicAdd(&tri_count[hit_tri], 1ull);
-                    atomicAdd(&cnt[kCntSettled], 1ull);
-                    break;
+                if (rr.outcome == kDeferNow) {
+                    ++nexact;
+                    double x0s[3];
+                    abs_pos(M, t_before, px, py, pz, x0s);
+                    settled = finish_exact(M, S, k, x0s, s, sink, hit_tri, exh, err);
                 }
+                if (rr.outcome == kSettledNow) {
+                    hit_tri = rr.hit_tri;
+                    settled = true;
+                }
+                if (settled) break;
             }
             if (extra) atomicAdd(&cnt[kCntProcSlot0 + (w & (kProcSlots - 1))], extra);
         }
+        if (settled) {
+            s.f = (s.f | kSettled) & ~kStrict;
+            P.hit[i] = hit_tri;
+            atomicAdd(&tri_count[hit_tri], 1ull);
+            atomicAdd(&cnt[kCntSettled], 1ull);
+        }
+        sink.flush(M);
         P.x[i] = s.x;
         P.y[i] = s.y;
         P.z[i] = s.z;
@@ -881,6 +942,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
         P.flags[i] = s.f;
         if (err) atomicOr(&cnt[kCntError], 1ull);
         if (exh) atomicAdd(&cnt[kCntExhaustive], (unsigned long long)exh);
+        if (nexact) atomicAdd(&cnt[kCntDeferred], (unsigned long long)nexact);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&cnt[kCntDeferred], (unsigned long long)nw);
 }
@@ -1004,7 +1066,7 @@ void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, cons
 {
     if (P.n > 0) step_main<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
-    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 256u));
+    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 64u));
     step_resolve<R><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
 }
 

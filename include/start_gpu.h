@@ -128,6 +128,15 @@ int start_load_particles(start_ctx* ctx, int64_t n, const double* x, const doubl
                          const double* z, const double* vx, const double* vy, const double* vz,
                          const int32_t* species, const uint64_t* ids);
 
+/* Replaces the background gas (model, target, concentration) and, when
+ * diameters is not NULL, the species' collision diameters (n_species values,
+ * r_p + r_t depends on the target); recomputes the collision constants.
+ * ParticleDynamicsProcessor::process takes the gas per call
+ * (ParticleDynamicsProcessor.hpp:167-169). */
+int start_set_gas(start_ctx* ctx, const start_gas_desc* gas, const double* diameters);
+/* Replaces the time step (ParticleDynamicsProcessor.hpp:157, >= 0). */
+int start_set_time_step(start_ctx* ctx, double dt);
+
 /* Per-tet field E (3*n_tets), as ChargeDensityEquationSolver::solve leaves it
  * (ChargeDensityEquationSolver.cpp:42, VolumetricMesh.cpp:382-411). */
 int start_set_tet_field(start_ctx* ctx, const double* E);
@@ -144,6 +153,23 @@ int start_step(start_ctx* ctx, int32_t nsteps);
 int start_gather_charge(start_ctx* ctx);
 
 int start_read_node_charge(start_ctx* ctx, double* out /* n_nodes */);
+
+/* Node charge density (C/m^3) as NodeChargeDensityProcessor::gather leaves it
+ * in NodeChargeDensitiesMap (NodeChargeDensityProcessor.cpp:60-93).
+ *   START_DENSITY_BARYCENTRIC: Q_n / V_n, Q_n the latest barycentric node
+ *     charge, V_n = (sum of the volumes of the tets around n) / 4; every node
+ *     is written.
+ *   START_DENSITY_TET_AVERAGE: the reference formula with one owner per
+ *     particle at the current positions: rho_t = sum q / V_t over the tets
+ *     owning at least one unsettled particle, rho_n = sum(rho_t V_t) / sum V_t
+ *     over those tets around n (ascending tet id); nodes with no such tet are
+ *     not written (written[n] = 0, rho[n] = 0), as the reference leaves their
+ *     map entries untouched.
+ * written may be NULL. */
+#define START_DENSITY_BARYCENTRIC 0
+#define START_DENSITY_TET_AVERAGE 1
+int start_node_charge_density(start_ctx* ctx, int32_t mode, double* rho /* n_nodes */,
+                              int32_t* written /* n_nodes or NULL */);
 int start_read_particles(start_ctx* ctx, double* x, double* y, double* z, double* vx, double* vy,
                          double* vz, int32_t* tet, int32_t* status, int32_t* hit_tri);
 int start_read_tri_counts(start_ctx* ctx, uint64_t* counts /* n_tris */);

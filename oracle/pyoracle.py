@@ -80,6 +80,9 @@ def lib() -> C.CDLL:
         L.or_step.restype = C.c_int
         L.or_scatter_charge.argtypes = [vp, P(SpeciesK), i64, P(d), P(d), P(d), P(i32), P(i32),
                                         P(i32), P(d)]
+        L.or_tet_volumes.argtypes = [vp, P(d)]
+        L.or_node_volumes.argtypes = [vp, P(d)]
+        L.or_node_density.argtypes = [vp, P(SpeciesK), i64, P(i32), P(i32), P(i32), P(d), P(i32)]
         _LIB = L
     return _LIB
 
@@ -193,6 +196,16 @@ class Mesh:
         lib().or_barycentric(self.h, int(t), _p(P, C.c_double), _p(lam, C.c_double))
         return lam
 
+    def tet_volumes(self) -> np.ndarray:
+        v = np.empty(self.n_tets)
+        lib().or_tet_volumes(self.h, _p(v, C.c_double))
+        return v
+
+    def node_volumes(self) -> np.ndarray:
+        v = np.empty(self.n_nodes)
+        lib().or_node_volumes(self.h, _p(v, C.c_double))
+        return v
+
     def surface_bbox(self, expansion=1.01):
         lo = np.empty(3)
         hi = np.empty(3)
@@ -259,3 +272,12 @@ class Sim:
                                 _p(st.y, C.c_double), _p(st.z, C.c_double), _p(st.tet, C.c_int32),
                                 _p(st.status, C.c_int32), _p(st.species, C.c_int32),
                                 _p(node_charge, C.c_double))
+
+    def node_density(self, st: State):
+        """Tet-average node charge density of the current owners; returns (rho, written)."""
+        rho = np.empty(self.mesh.n_nodes)
+        wr = np.empty(self.mesh.n_nodes, dtype=np.int32)
+        lib().or_node_density(self.mesh.h, self.sp, len(st.x), _p(st.tet, C.c_int32),
+                              _p(st.status, C.c_int32), _p(st.species, C.c_int32),
+                              _p(rho, C.c_double), _p(wr, C.c_int32))
+        return rho, wr.astype(bool)

@@ -714,6 +714,81 @@ void or_scatter_charge(const or_mesh* m, const or_species_k* sp, int64_t n, cons
     }
 }
 
+/* Tet volume ((b-a) . ((c-a) x (d-a)))/6 of the positively oriented tet: the
+ * value CGAL's Tetrahedron_3::volume gives behind
+ * getVolumeByGlobalTetraId (VolumetricMesh.hpp:185), up to its own rounding. */
+static double tet_volume(const or_mesh* m, int64_t t)
+{
+    const int32_t* v = m->tets + 4 * t;
+    const double *a = m->nodes + 3 * v[0], *b = m->nodes + 3 * v[1], *c = m->nodes + 3 * v[2],
+                 *d = m->nodes + 3 * v[3];
+    double u[3], w[3], s[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = b[k] - a[k];
+        w[k] = c[k] - a[k];
+        s[k] = d[k] - a[k];
+    }
+    double cx = w[1] * s[2] - w[2] * s[1];
+    double cy = w[2] * s[0] - w[0] * s[2];
+    double cz = w[0] * s[1] - w[1] * s[0];
+    return ((u[0] * cx + u[1] * cy) + u[2] * cz) / 6.0;
+}
+
+void or_tet_volumes(const or_mesh* m, double* vol)
+{
+    for (int64_t t = 0; t < m->nt; ++t) vol[t] = tet_volume(m, t);
+}
+
+/* Lumped node volume V_n = (sum of the volumes of the tets around n, in
+ * ascending tet id) / 4. */
+void or_node_volumes(const or_mesh* m, double* vol)
+{
+    memset(vol, 0, sizeof(double) * (size_t)m->nn);
+    for (int64_t t = 0; t < m->nt; ++t) {
+        double v = tet_volume(m, t);
+        for (int j = 0; j < 4; ++j) vol[m->tets[4 * t + j]] += v;
+    }
+    for (int64_t n = 0; n < m->nn; ++n) vol[n] = vol[n] / 4.0;
+}
+
+/* NodeChargeDensityProcessor::_gather_stdver__helper, one thread
+ * (NodeChargeDensityProcessor.cpp:51-93) with a single owner per particle:
+ *   rho_t = (0 + q_1 + q_2 + ...) / V_t over the unsettled particles owned by t
+ *           (particle order), for every tet owning at least one particle;
+ *   rho_n = (sum rho_t * V_t) / (sum V_t) over the occupied tets around n
+ *           (ascending tet id), written only where that volume is > 0. */
+void or_node_density(const or_mesh* m, const or_species_k* sp, int64_t n, const int32_t* tet,
+                     const int32_t* status, const int32_t* species, double* rho, int32_t* written)
+{
+    double* tq = (double*)calloc((size_t)m->nt, sizeof(double));
+    int32_t* tn = (int32_t*)calloc((size_t)m->nt, sizeof(int32_t));
+    for (int64_t i = 0; i < n; ++i) {
+        if (status[i] == OR_SETTLED || tet[i] < 0) continue;
+        tq[tet[i]] = tq[tet[i]] + sp[species[i]].charge;
+        tn[tet[i]] += 1;
+    }
+    double* tc = (double*)calloc((size_t)m->nn, sizeof(double));
+    double* tv = (double*)calloc((size_t)m->nn, sizeof(double));
+    for (int64_t t = 0; t < m->nt; ++t) {
+        if (!tn[t]) continue;
+        double v = tet_volume(m, t);
+        double rt = tq[t] / v;
+        for (int j = 0; j < 4; ++j) {
+            int32_t nd = m->tets[4 * t + j];
+            tc[nd] += rt * v;
+            tv[nd] += v;
+        }
+    }
+    for (int64_t k = 0; k < m->nn; ++k) {
+        written[k] = tv[k] > 0.0;
+        rho[k] = tv[k] > 0.0 ? tc[k] / tv[k] : 0.0;
+    }
+    free(tq);
+    free(tn);
+    free(tc);
+    free(tv);
+}
+
 /* ------------------------------------------------------------------------- */
 /* One step over all particles (ParticleDynamicsProcessor.cpp:161-200), then  */
 /* relocation and charge for the next step's gather                          */

@@ -111,6 +111,12 @@ void or_scatter_charge(const or_mesh* m, const or_species_k* sp, int64_t n, cons
                        const double* y, const double* z, const int32_t* tet, const int32_t* status,
                        const int32_t* species, double* node_charge);
 
+/* ---- node charge density (NodeChargeDensityProcessor.cpp:51-93) ---- */
+void or_tet_volumes(const or_mesh* m, double* vol);
+void or_node_volumes(const or_mesh* m, double* vol);
+void or_node_density(const or_mesh* m, const or_species_k* sp, int64_t n, const int32_t* tet,
+                     const int32_t* status, const int32_t* species, double* rho, int32_t* written);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,9 @@ EXPORTS = [
     "start_gather_charge", "start_read_node_charge", "start_read_particles",
     "start_read_tri_counts", "start_read_settled", "start_read_stats", "start_synchronize",
     "start_locate", "start_orient3d_batch", "start_timing_enable", "start_timing_read",
+    "start_node_charge_density", "start_set_gas", "start_set_time_step",
 ]
+DENSITY = {"barycentric": 0, "tet_average": 1}
 
 
 class MeshDesc(C.Structure):
@@ -90,6 +92,9 @@ def lib() -> C.CDLL:
     L.start_orient3d_batch.argtypes = [i64, P(d), P(i32)]
     L.start_timing_enable.argtypes = [vp, i32]
     L.start_timing_read.argtypes = [vp, P(d), P(u64)]
+    L.start_node_charge_density.argtypes = [vp, i32, P(d), P(i32)]
+    L.start_set_gas.argtypes = [vp, P(GasDesc), P(d)]
+    L.start_set_time_step.argtypes = [vp, d]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is C.c_int:  # default

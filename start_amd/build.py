@@ -43,5 +43,34 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libstart_host.so")
+SHIM_CHECK = os.path.join(HERE, "shim_check")
+
+
+def build_host(verbose: bool = False) -> list[str]:
+    """The C++ step-loop entry points (libstart_host.so) over libstart_gpu.so,
+    and the shim_check driver that runs them like the reference's main loop."""
+    gxx = shutil.which("g++") or "g++"
+    common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-I", HOST]
+    cmds = [
+        [gxx] + common + ["-fPIC", "-shared", "-o", HOST_LIB + ".tmp", os.path.join(HOST, "StartProcessors.cpp"),
+                          "-L", HERE, "-lstart_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"],
+        [gxx] + common + ["-o", SHIM_CHECK + ".tmp", os.path.join(HOST, "shim_check.cpp"),
+                          "-L", HERE, "-lstart_host", "-lstart_gpu", "-Wl,-rpath,$ORIGIN"],
+    ]
+    outs = [HOST_LIB, SHIM_CHECK]
+    for cmd, out in zip(cmds, outs):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed building {os.path.basename(out)}")
+        os.replace(out + ".tmp", out)
+    return outs
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_host(verbose=True))

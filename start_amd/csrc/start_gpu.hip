@@ -24,6 +24,11 @@ void launch_locate(hipStream_t s, const DevMesh& M, const double* x, const doubl
 template <typename R>
 void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,
                    double* node_charge);
+template <typename R>
+void launch_node_density(hipStream_t s, const DevParticles<R>& P, const SpeciesK* sp, int32_t mode,
+                         const double* node_q, const double* node_vol, const int32_t* adj_start,
+                         const int32_t* adj_list, const double* tet_vol, double* tet_q, int32_t* tet_n,
+                         int64_t n_nodes, double* rho, int32_t* written);
 void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32);
 void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
@@ -61,6 +66,7 @@ struct start_ctx {
     hipStream_t stream = nullptr;
     HostMesh hm;
     start_params prm{};
+    std::vector<start_species_desc> species_desc;
     bool mixed = false;
     size_t es = sizeof(double);  // bytes of one position/velocity component
     std::vector<SpeciesK> species;
@@ -82,6 +88,9 @@ struct start_ctx {
     int wslot = 0;                    // work-list counter slot of the next launch
     SpeciesK* d_species = nullptr;
     float tau32 = 0.0f;
+    // node charge density tables (built on first use)
+    double *d_tet_vol = nullptr, *d_node_vol = nullptr, *d_tet_q = nullptr;
+    int32_t *d_adj_start = nullptr, *d_adj_list = nullptr, *d_tet_n = nullptr;
     // particles
     int64_t n = 0;
     PBuf A, B;  // live arrays and the sort target
@@ -268,6 +277,65 @@ float build_geo32(const HostMesh& hm, std::vector<float>& g32)
     return (float)tau;
 }
 
+// Tet volumes ((b-a) . ((c-a) x (d-a)))/6 of the positively oriented tets,
+// lumped node volumes (sum over the tets around a node, ascending tet id, / 4)
+// and the node -> tets adjacency in ascending tet id.
+int ensure_density_tables(start_ctx* c)
+{
+    if (c->d_tet_vol) return START_OK;
+    const HostMesh& hm = c->hm;
+    const int64_t T = hm.n_tets, M = hm.n_nodes;
+    std::vector<double> tv((size_t)T), nv((size_t)M, 0.0);
+    std::vector<int32_t> start((size_t)M + 1, 0), list((size_t)(4 * T));
+    for (int64_t t = 0; t < T; ++t) {
+        const int32_t* v = &hm.tets[4 * (size_t)t];
+        const double *a = &hm.nodes[3 * (size_t)v[0]], *b = &hm.nodes[3 * (size_t)v[1]],
+                     *cc = &hm.nodes[3 * (size_t)v[2]], *d = &hm.nodes[3 * (size_t)v[3]];
+        double u[3], w[3], s[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = b[k] - a[k];
+            w[k] = cc[k] - a[k];
+            s[k] = d[k] - a[k];
+        }
+        double cx = w[1] * s[2] - w[2] * s[1];
+        double cy = w[2] * s[0] - w[0] * s[2];
+        double cz = w[0] * s[1] - w[1] * s[0];
+        double vol = ((u[0] * cx + u[1] * cy) + u[2] * cz) / 6.0;
+        tv[(size_t)t] = vol;
+        for (int j = 0; j < 4; ++j) {
+            nv[(size_t)v[j]] += vol;
+            start[(size_t)v[j] + 1] += 1;
+        }
+    }
+    for (int64_t n = 0; n < M; ++n) {
+        nv[(size_t)n] = nv[(size_t)n] / 4.0;
+        start[(size_t)n + 1] += start[(size_t)n];
+    }
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    for (int64_t t = 0; t < T; ++t)
+        for (int j = 0; j < 4; ++j) list[(size_t)fill[(size_t)hm.tets[4 * (size_t)t + j]]++] = (int32_t)t;
+    hipError_t e = dalloc(&c->d_tet_vol, T);
+    if (e == hipSuccess) e = dalloc(&c->d_node_vol, M);
+    if (e == hipSuccess) e = dalloc(&c->d_tet_q, T);
+    if (e == hipSuccess) e = dalloc(&c->d_tet_n, T);
+    if (e == hipSuccess) e = dalloc(&c->d_adj_start, M + 1);
+    if (e == hipSuccess) e = dalloc(&c->d_adj_list, 4 * T);
+    if (e == hipSuccess) e = hipMemcpy(c->d_tet_vol, tv.data(), sizeof(double) * T, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_node_vol, nv.data(), sizeof(double) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_adj_start, start.data(), sizeof(int32_t) * (M + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_adj_list, list.data(), sizeof(int32_t) * 4 * T, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        void* ptrs[] = {c->d_tet_vol, c->d_node_vol, c->d_tet_q, c->d_tet_n, c->d_adj_start, c->d_adj_list};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        c->d_tet_vol = c->d_node_vol = c->d_tet_q = nullptr;
+        c->d_tet_n = c->d_adj_start = c->d_adj_list = nullptr;
+        return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("density tables: ") + hipGetErrorString(e));
+    }
+    return START_OK;
+}
+
 int check_kernel_errors(start_ctx* c)
 {
     unsigned long long h[kCntTotal];
@@ -318,7 +386,10 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         delete c;
         return fail(nullptr, START_ERR_MESH, e);
     }
-    for (int s = 0; s < params->n_species; ++s) c->species.push_back(species_constants(params->species[s], params->gas));
+    for (int s = 0; s < params->n_species; ++s) {
+        c->species_desc.push_back(params->species[s]);
+        c->species.push_back(species_constants(params->species[s], params->gas));
+    }
     std::vector<float> g32;
     if (c->mixed) {
         c->tau32 = build_geo32(c->hm, g32);
@@ -398,9 +469,10 @@ void start_destroy(start_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_particles(c);
-    void* ptrs[] = {c->d_nodes, c->d_geo, c->d_geo32, c->d_E, c->d_E32, c->d_tnode, c->d_tnbr, c->d_tris,
-                    c->d_tri_deg, c->d_tgs, c->d_tgl, c->d_sgs, c->d_sgl, c->d_tri_count, c->d_node_charge,
-                    c->d_species, c->d_cnt, c->d_nactive};
+    void* ptrs[] = {c->d_nodes,     c->d_geo,      c->d_geo32,    c->d_E,         c->d_E32,     c->d_tnode,
+                    c->d_tnbr,      c->d_tris,     c->d_tri_deg,  c->d_tgs,       c->d_tgl,     c->d_sgs,
+                    c->d_sgl,       c->d_tri_count, c->d_node_charge, c->d_species, c->d_cnt,  c->d_nactive,
+                    c->d_tet_vol,   c->d_node_vol, c->d_tet_q,    c->d_adj_start, c->d_adj_list, c->d_tet_n};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->events) {
@@ -515,6 +587,35 @@ int start_load_particles(start_ctx* c, int64_t n, const double* x, const double*
         if (rc != START_OK) return rc;
     }
     return check_kernel_errors(c);
+}
+
+int start_set_gas(start_ctx* c, const start_gas_desc* g, const double* diameters)
+{
+    if (!c || !g) return fail(c, START_ERR_ARG, "null argument");
+    if (g->model < START_MODEL_NONE || g->model > START_MODEL_VSS) return fail(c, START_ERR_ARG, "unknown collision model");
+    if (!(g->concentration >= 0.0)) return fail(c, START_ERR_ARG, "negative gas concentration");
+    std::vector<start_species_desc> sd = c->species_desc;
+    if (diameters)
+        for (size_t s = 0; s < sd.size(); ++s) {
+            if (!(diameters[s] >= 0.0)) return fail(c, START_ERR_ARG, "negative collision diameter");
+            sd[s].diameter = diameters[s];
+        }
+    std::vector<SpeciesK> k;
+    for (const auto& s : sd) k.push_back(species_constants(s, *g));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(c->d_species, k.data(), sizeof(SpeciesK) * k.size(), hipMemcpyHostToDevice));
+    c->species_desc = sd;
+    c->species = k;
+    c->prm.gas = *g;
+    return START_OK;
+}
+
+int start_set_time_step(start_ctx* c, double dt)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (!(dt >= 0.0)) return fail(c, START_ERR_ARG, "time step must be >= 0");
+    c->prm.dt = dt;
+    return START_OK;
 }
 
 int start_set_tet_field(start_ctx* c, const double* E)
@@ -633,6 +734,44 @@ int start_read_node_charge(start_ctx* c, double* out)
                                sizeof(double) * c->hm.n_nodes, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return check_kernel_errors(c);
+}
+
+int start_node_charge_density(start_ctx* c, int32_t mode, double* rho, int32_t* written)
+{
+    if (!c || !rho) return fail(c, START_ERR_ARG, "null argument");
+    if (mode != START_DENSITY_BARYCENTRIC && mode != START_DENSITY_TET_AVERAGE)
+        return fail(c, START_ERR_ARG, "unknown density mode");
+    int rc = ensure_density_tables(c);
+    if (rc != START_OK) return rc;
+    const int64_t M = c->hm.n_nodes, T = c->hm.n_tets;
+    double* d_rho = nullptr;
+    int32_t* d_wr = nullptr;
+    hipError_t e = dalloc(&d_rho, M);
+    if (e == hipSuccess && written) e = dalloc(&d_wr, M);
+    if (e == hipSuccess && mode == START_DENSITY_TET_AVERAGE) {
+        e = hipMemsetAsync(c->d_tet_q, 0, sizeof(double) * T, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(c->d_tet_n, 0, sizeof(int32_t) * T, c->stream);
+    }
+    if (e == hipSuccess) {
+        const double* q = c->d_node_charge + (size_t)c->latest * M;
+        if (c->mixed)
+            launch_node_density<float>(c->stream, c->view<float>(), c->d_species, mode, q, c->d_node_vol,
+                                       c->d_adj_start, c->d_adj_list, c->d_tet_vol, c->d_tet_q, c->d_tet_n, M, d_rho,
+                                       d_wr);
+        else
+            launch_node_density<double>(c->stream, c->view<double>(), c->d_species, mode, q, c->d_node_vol,
+                                        c->d_adj_start, c->d_adj_list, c->d_tet_vol, c->d_tet_q, c->d_tet_n, M, d_rho,
+                                        d_wr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(rho, d_rho, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && written)
+        e = hipMemcpyAsync(written, d_wr, sizeof(int32_t) * M, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (d_rho) (void)hipFree(d_rho);
+    if (d_wr) (void)hipFree(d_wr);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("node charge density: ") + hipGetErrorString(e));
+    return START_OK;
 }
 
 int start_read_particles(start_ctx* c, double* x, double* y, double* z, double* vx, double* vy, double* vz,

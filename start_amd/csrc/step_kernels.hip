@@ -994,6 +994,51 @@ __global__ void gather_kernel(DevMesh M, DevParticles<R> P, const SpeciesK* __re
     scatter4(node_charge, M.tnode[t], q, lam);
 }
 
+// Occupancy and summed charge of each tet (one owner per unsettled particle).
+template <typename R>
+__global__ void tet_occupancy_kernel(DevParticles<R> P, const SpeciesK* __restrict__ sp, double* __restrict__ tet_q,
+                                     int32_t* __restrict__ tet_n)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    int32_t f = P.flags[i], t = P.tet[i];
+    if ((f & kSettled) || t < 0) return;
+    atomicAdd(&tet_n[t], 1);
+    double q = sp[(f >> kSpeciesShift) & 0xff].charge;
+    if (q != 0.0) unsafeAtomicAdd(&tet_q[t], q);
+}
+
+// rho_n = sum(rho_t V_t) / sum V_t over the occupied tets around n, ascending tet id
+// (NodeChargeDensityProcessor.cpp:69-93).
+__global__ void node_density_kernel(const int32_t* __restrict__ adj_start, const int32_t* __restrict__ adj_list,
+                                    const double* __restrict__ tet_vol, const double* __restrict__ tet_q,
+                                    const int32_t* __restrict__ tet_n, int64_t n_nodes, double* __restrict__ rho,
+                                    int32_t* __restrict__ written)
+{
+    int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    double tc = 0.0, tv = 0.0;
+    for (int32_t j = adj_start[n]; j < adj_start[n + 1]; ++j) {
+        int32_t t = adj_list[j];
+        if (tet_n[t] == 0) continue;
+        double v = tet_vol[t];
+        double rt = tet_q[t] / v;
+        tc += rt * v;
+        tv += v;
+    }
+    rho[n] = tv > 0.0 ? tc / tv : 0.0;
+    if (written) written[n] = tv > 0.0;
+}
+
+__global__ void node_density_bary_kernel(const double* __restrict__ q, const double* __restrict__ node_vol,
+                                         int64_t n_nodes, double* __restrict__ rho, int32_t* __restrict__ written)
+{
+    int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    rho[n] = q[n] / node_vol[n];
+    if (written) written[n] = 1;
+}
+
 // E = -grad(phi) = -sum_j (phi_j - phi_0) A_j  (A_j: gradient of lambda_j)
 __global__ void field_kernel(DevMesh M, const double* __restrict__ phi, double* __restrict__ E, float* __restrict__ E32)
 {
@@ -1081,6 +1126,27 @@ void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, co
 {
     if (P.n > 0) gather_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, sp, node_charge);
 }
+
+template <typename R>
+void launch_node_density(hipStream_t s, const DevParticles<R>& P, const SpeciesK* sp, int32_t mode,
+                         const double* node_q, const double* node_vol, const int32_t* adj_start,
+                         const int32_t* adj_list, const double* tet_vol, double* tet_q, int32_t* tet_n,
+                         int64_t n_nodes, double* rho, int32_t* written)
+{
+    if (mode == 0) {
+        node_density_bary_kernel<<<blocks_for(n_nodes, 256), 256, 0, s>>>(node_q, node_vol, n_nodes, rho, written);
+        return;
+    }
+    if (P.n > 0) tet_occupancy_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(P, sp, tet_q, tet_n);
+    node_density_kernel<<<blocks_for(n_nodes, 256), 256, 0, s>>>(adj_start, adj_list, tet_vol, tet_q, tet_n, n_nodes,
+                                                                  rho, written);
+}
+template void launch_node_density<double>(hipStream_t, const DevParticles<double>&, const SpeciesK*, int32_t,
+                                          const double*, const double*, const int32_t*, const int32_t*,
+                                          const double*, double*, int32_t*, int64_t, double*, int32_t*);
+template void launch_node_density<float>(hipStream_t, const DevParticles<float>&, const SpeciesK*, int32_t,
+                                         const double*, const double*, const int32_t*, const int32_t*,
+                                         const double*, double*, int32_t*, int64_t, double*, int32_t*);
 
 void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32)
 {

@@ -110,6 +110,15 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_read_node_charge(self.ctx, _dp(out)))
         return out
 
+    def node_charge_density(self, mode: str = "barycentric"):
+        """Node charge density (C/m^3) and the written-node mask, as gather leaves
+        NodeChargeDensitiesMap: "barycentric" (Q_n over the lumped node volume) or
+        "tet_average" (the reference formula, NodeChargeDensityProcessor.cpp:60-93)."""
+        rho = np.empty(self.mesh.n_nodes, dtype=np.float64)
+        wr = np.empty(self.mesh.n_nodes, dtype=np.int32)
+        self._check(_lib.lib().start_node_charge_density(self.ctx, _lib.DENSITY[mode], _dp(rho), _ip(wr)))
+        return rho, wr.astype(bool)
+
     def particles(self) -> dict:
         n = self.n
         f = {k: np.empty(n, dtype=np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}

@@ -1,0 +1,329 @@
+// Host side of the step loop's two entry points over libstart_gpu's C ABI.
+#include "StartProcessors.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <iostream>
+#include <utility>
+
+namespace start_amd {
+
+namespace {
+
+constexpr double kECharge = 1.602176565e-19;  // Constants.hpp:14
+
+double f32(double v) { return (double)(float)v; }
+
+struct TypeRow {
+    const char* name;
+    TypeProperties p;
+};
+
+// Constants.hpp:20-73 (masses, radii, VTI omega and VSS alpha as float
+// constants); charges ParticlePropertiesManager.hpp:191-217.
+const TypeRow kTypes[] = {
+    {"O2", {53.1e-27, 2.035e-10, 0.0, f32(0.77), f32(1.40)}},
+    {"Ar", {6.6335209e-26, 71e-12, 0.0, f32(0.81), f32(1.40)}},
+    {"Ne", {3.3509177e-26, 160e-12, 0.0, f32(0.66), f32(1.31)}},
+    {"He", {6.6464731e-27, 120e-12, 0.0, f32(0.66), f32(1.26)}},
+    {"Ti", {7.9485017e-26, 140e-12, 2 * kECharge, 0.0, 0.0}},
+    {"Al", {4.4803831e-26, 125e-12, 3 * kECharge, 0.0, 0.0}},
+    {"Sn", {1.9712258e-25, 145e-12, 2 * kECharge, 0.0, 0.0}},
+    {"W", {8.4590343e-26, 135e-12, 6 * kECharge, 0.0, 0.0}},
+    {"Au", {3.2707137e-25, 135e-12, 3 * kECharge, 0.0, 0.0}},
+    {"Cu", {1.0552061e-25, 135e-12, 1 * kECharge, 0.0, 0.0}},
+    {"Ni", {9.7462675e-26, 135e-12, 2 * kECharge, 0.0, 0.0}},
+    {"Ag", {1.7911901e-25, 160e-12, 1 * kECharge, 0.0, 0.0}},
+};
+constexpr int kNumTypes = (int)Unknown;
+static_assert(sizeof(kTypes) / sizeof(kTypes[0]) == (size_t)Unknown, "one row per particle type");
+
+std::string upper(std::string_view s)
+{
+    std::string u(s);
+    for (auto& ch : u) ch = (char)std::toupper((unsigned char)ch);
+    return u;
+}
+
+// CollisionModelFactory::create dispatches on the upper-cased name (CollisionModelFactory.cpp:23-36).
+int modelCode(std::string_view name)
+{
+    std::string u = upper(name);
+    if (u == "HS") return START_MODEL_HS;
+    if (u == "VHS") return START_MODEL_VHS;
+    if (u == "VSS") return START_MODEL_VSS;
+    throw std::runtime_error("unknown scattering model '" + std::string(name) + "'");
+}
+
+}  // namespace
+
+TypeProperties typeProperties(ParticleType t)
+{
+    if (t < 0 || t >= Unknown) throw std::runtime_error("unknown particle type");
+    return kTypes[t].p;
+}
+
+ParticleType typeFromName(std::string_view name)
+{
+    for (int t = 0; t < kNumTypes; ++t)
+        if (name == kTypes[t].name) return (ParticleType)t;
+    return Unknown;
+}
+
+Particle::Particle(size_t id, ParticleType type, double x, double y, double z, double vx, double vy, double vz)
+    : m_id(id), m_type(type), m_centre{x, y, z}, m_velocity{0.0, 0.0, 0.0}, m_energy(0.0)
+{
+    setVelocity(vx, vy, vz);
+}
+
+double Particle::getMass() const { return typeProperties(m_type).mass; }
+double Particle::getCharge() const { return typeProperties(m_type).charge; }
+
+void Particle::setVelocity(double vx, double vy, double vz)
+{
+    m_velocity = {vx, vy, vz};
+    m_energy = 0.5 * getMass() * (vx * vx + vy * vy + vz * vz);
+}
+
+// ---------------------------------------------------------------------------
+
+ParticleKernelDevice::ParticleKernelDevice(DeviceMeshInput mesh, DeviceKernelOptions options)
+    : m_mesh(std::move(mesh)), m_opt(std::move(options))
+{
+    if (m_mesh.nodes.size() % 3 || m_mesh.tets.size() % 4 || m_mesh.triangles.size() % 3)
+        throw std::invalid_argument("mesh arrays are not whole rows");
+    size_t M = m_mesh.nodes.size() / 3, S = m_mesh.triangles.size() / 3;
+    if (m_mesh.nodeTags.empty())
+        for (size_t n = 0; n < M; ++n) m_mesh.nodeTags.push_back(n + 1);  // Gmsh tags 1..M (Q19)
+    if (m_mesh.triangleTags.empty())
+        for (size_t s = 0; s < S; ++s) m_mesh.triangleTags.push_back(s);
+    if (m_mesh.nodeTags.size() != M || m_mesh.triangleTags.size() != S)
+        throw std::invalid_argument("tag arrays do not match the mesh");
+}
+
+ParticleKernelDevice::~ParticleKernelDevice()
+{
+    if (m_ctx) start_destroy(m_ctx);
+}
+
+void ParticleKernelDevice::check(int rc, const char* what)
+{
+    if (rc != START_OK) throw std::runtime_error(std::string(what) + ": " + start_last_error(m_ctx));
+}
+
+void ParticleKernelDevice::ensureContext()
+{
+    if (m_ctx) return;
+    std::vector<start_species_desc> sp((size_t)kNumTypes);
+    for (int t = 0; t < kNumTypes; ++t) {
+        TypeProperties p = kTypes[t].p;
+        auto q = m_opt.chargeOverride.find((ParticleType)t);
+        sp[(size_t)t] = start_species_desc{p.mass, q != m_opt.chargeOverride.end() ? q->second : p.charge, 0.0};
+    }
+    start_mesh_desc md{};
+    md.n_nodes = (int64_t)(m_mesh.nodes.size() / 3);
+    md.nodes = m_mesh.nodes.data();
+    md.n_tets = (int64_t)(m_mesh.tets.size() / 4);
+    md.tets = m_mesh.tets.data();
+    md.n_tris = (int64_t)(m_mesh.triangles.size() / 3);
+    md.tris = m_mesh.triangles.data();
+    start_params prm{};
+    prm.dt = m_dt;
+    prm.bbox_expansion = m_opt.bboxExpansion;
+    prm.seed = m_opt.seed;
+    prm.run = m_opt.run;
+    prm.em_push = m_opt.emPush ? 1 : 0;
+    prm.precision = m_opt.precision;
+    prm.charge_every_step = 1;
+    prm.n_species = kNumTypes;
+    prm.steps_per_launch = 1;
+    prm.species = sp.data();
+    prm.gas = start_gas_desc{0.0, 0.0, 0.0, 0.0, START_MODEL_NONE, 0};
+    prm.sort_interval = m_opt.sortInterval;
+    start_ctx* c = nullptr;
+    int rc = start_create(&c, &md, &prm);
+    if (rc != START_OK) throw std::runtime_error(std::string("device set-up: ") + start_last_error(nullptr));
+    m_ctx = c;
+    if (!m_E.empty()) check(start_set_tet_field(m_ctx, m_E.data()), "tet field");
+    if (!m_phi_rows.empty()) check(start_set_node_potential(m_ctx, m_phi_rows.data()), "node potential");
+}
+
+void ParticleKernelDevice::configure(std::string_view scatteringModel, std::string_view gasName,
+                                     double gasConcentration, double timeStep)
+{
+    if (m_configured && m_model == scatteringModel && m_gas == gasName && m_conc == gasConcentration &&
+        m_dt == timeStep)
+        return;
+    int code = modelCode(scatteringModel);
+    ParticleType g = typeFromName(gasName);
+    if (g == Unknown) throw std::runtime_error("unknown gas '" + std::string(gasName) + "'");
+    TypeProperties gp = typeProperties(g);
+    start_gas_desc gd{gp.mass, gp.omega, gp.alpha, gasConcentration, code, 0};
+    std::vector<double> diam((size_t)kNumTypes);
+    for (int t = 0; t < kNumTypes; ++t) {
+        auto d = m_opt.diameterOverride.find((ParticleType)t);
+        diam[(size_t)t] = d != m_opt.diameterOverride.end() ? d->second : kTypes[t].p.radius + gp.radius;
+    }
+    check(start_set_gas(m_ctx, &gd, diam.data()), "gas");
+    check(start_set_time_step(m_ctx, timeStep), "time step");
+    m_model = std::string(scatteringModel);
+    m_gas = std::string(gasName);
+    m_conc = gasConcentration;
+    m_dt = timeStep;
+    m_configured = true;
+}
+
+void ParticleKernelDevice::ensureParticles(const ParticleVector& particles)
+{
+    if (m_n == particles.size() && m_n > 0) return;
+    size_t n = particles.size();
+    std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n);
+    std::vector<int32_t> sp(n);
+    std::vector<uint64_t> ids(n);
+    for (size_t i = 0; i < n; ++i) {
+        const Particle& p = particles[i];
+        if (p.getType() == Unknown) throw std::runtime_error("particle of unknown type");
+        auto c = p.getCentre();
+        auto v = p.getVelocity();
+        x[i] = c[0];
+        y[i] = c[1];
+        z[i] = c[2];
+        vx[i] = v[0];
+        vy[i] = v[1];
+        vz[i] = v[2];
+        sp[i] = (int32_t)p.getType();
+        ids[i] = (uint64_t)p.getId();
+    }
+    check(start_load_particles(m_ctx, (int64_t)n, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
+                               sp.data(), ids.data()),
+          "particle upload");
+    m_n = n;
+    m_settled_seen = 0;
+    m_charge_fresh = false;
+}
+
+void ParticleKernelDevice::step(ParticleVector& particles, ParticlesIDSet& settled, StopSubject& stop)
+{
+    check(start_step(m_ctx, 1), "step");
+    ++m_steps;
+    m_charge_fresh = true;  // the step scatters the end-of-step charge
+    start_stats s{};
+    check(start_read_stats(m_ctx, &s), "step statistics");
+    if (s.settled != m_settled_seen) {
+        std::vector<uint64_t> ids((size_t)s.settled);
+        int64_t k = (int64_t)ids.size();
+        check(start_read_settled(m_ctx, ids.data(), &k), "settled ids");
+        settled.insert(ids.begin(), ids.begin() + k);
+        m_settled_seen = s.settled;
+    }
+    if (s.stop) stop.notifyStopRequested();  // ParticleSettler.cpp:24-25
+    if (m_opt.writeBackEachStep) syncParticles(particles);
+}
+
+void ParticleKernelDevice::syncParticles(ParticleVector& particles)
+{
+    if (!m_ctx || m_n == 0) return;
+    if (particles.size() != m_n) throw std::runtime_error("particle vector size differs from the device's");
+    std::vector<double> x(m_n), y(m_n), z(m_n), vx(m_n), vy(m_n), vz(m_n);
+    check(start_read_particles(m_ctx, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(), nullptr,
+                               nullptr, nullptr),
+          "particle read-back");
+    for (size_t i = 0; i < m_n; ++i) {
+        particles[i].setCentre(x[i], y[i], z[i]);
+        particles[i].setVelocity(vx[i], vy[i], vz[i]);
+    }
+}
+
+void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
+{
+    if (m_opt.densityMode == START_DENSITY_BARYCENTRIC && !m_charge_fresh) {
+        check(start_gather_charge(m_ctx), "charge scatter");
+        m_charge_fresh = true;
+    }
+    size_t M = m_mesh.nodes.size() / 3;
+    std::vector<double> rho(M);
+    std::vector<int32_t> wr(M);
+    check(start_node_charge_density(m_ctx, m_opt.densityMode, rho.data(), wr.data()), "node charge density");
+    for (size_t n = 0; n < M; ++n)
+        if (wr[n]) out[m_mesh.nodeTags[n]] = rho[n];
+}
+
+void ParticleKernelDevice::setNodePotential(const std::map<size_t, double>& potentialByTag)
+{
+    size_t M = m_mesh.nodes.size() / 3;
+    std::map<size_t, size_t> row;
+    for (size_t n = 0; n < M; ++n) row[m_mesh.nodeTags[n]] = n;
+    std::vector<double> phi(M, 0.0);
+    for (const auto& [tag, v] : potentialByTag) {
+        auto r = row.find(tag);
+        if (r == row.end()) throw std::invalid_argument("potential for an unknown node tag");
+        phi[r->second] = v;
+    }
+    m_E.clear();
+    if (m_ctx) check(start_set_node_potential(m_ctx, phi.data()), "node potential");
+    else m_phi_rows = std::move(phi);
+}
+
+void ParticleKernelDevice::setTetField(const std::vector<double>& E)
+{
+    if (E.size() != m_mesh.tets.size() / 4 * 3) throw std::invalid_argument("field must hold 3 values per tet");
+    m_phi_rows.clear();
+    if (m_ctx) check(start_set_tet_field(m_ctx, E.data()), "tet field");
+    else m_E = E;
+}
+
+std::map<size_t, uint64_t> ParticleKernelDevice::triangleCounts()
+{
+    std::map<size_t, uint64_t> out;
+    size_t S = m_mesh.triangles.size() / 3;
+    if (!m_ctx) {
+        for (size_t s = 0; s < S; ++s) out[m_mesh.triangleTags[s]] = 0;
+        return out;
+    }
+    std::vector<uint64_t> c(std::max<size_t>(S, 1));
+    check(start_read_tri_counts(m_ctx, c.data()), "triangle counts");
+    for (size_t s = 0; s < S; ++s) out[m_mesh.triangleTags[s]] = c[s];
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+
+void ParticleDynamicsProcessor::process(std::string_view /*config_filename*/, ParticleVector& particles,
+                                        double /*timeMoment*/, double timeStep, unsigned int /*numThreads*/,
+                                        ParticleKernelDevice& device, ParticleTrackerMap& /*particleTracker*/,
+                                        ParticlesIDSet& settledParticlesIds,
+                                        std::shared_mutex& /*sh_mutex_settledParticlesCounterMap*/,
+                                        std::mutex& /*mutex_particlesMovementMap*/,
+                                        ParticleMovementMap& /*particleMovementMap*/, StopSubject& stopSubject,
+                                        std::string_view scatteringModel, std::string_view gasName,
+                                        double gasConcentration)
+{
+    try {
+        device.ensureContext();
+        device.configure(scatteringModel, gasName, gasConcentration, timeStep);
+        device.ensureParticles(particles);
+        device.step(particles, settledParticlesIds, stopSubject);
+    } catch (std::exception const& ex) {
+        std::cerr << "ERROR: particle step on the GPU failed: " << ex.what() << std::endl;
+    } catch (...) {
+        std::cerr << "ERROR: particle step on the GPU failed" << std::endl;
+    }
+}
+
+void NodeChargeDensityProcessor::gather(double /*timeMoment*/, std::string_view /*configFilename*/,
+                                        ParticleKernelDevice& device, ParticleVector const& particles,
+                                        ParticlesIDSet& /*settledParticleIDs*/,
+                                        ParticleTrackerMap& /*particleTrackerMap*/,
+                                        NodeChargeDensitiesMap& nodeChargeDensityMap)
+{
+    try {
+        device.ensureContext();
+        device.ensureParticles(particles);
+        device.densities(nodeChargeDensityMap);
+    } catch (std::exception const& ex) {
+        throw NodeChargeDensityException(std::string("gathering node charge densities failed: ") + ex.what());
+    }
+}
+
+}  // namespace start_amd

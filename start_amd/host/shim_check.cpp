@@ -1,0 +1,144 @@
+// Drives ParticleDynamicsProcessor::process and NodeChargeDensityProcessor::gather
+// the way Start's step loop does (ModelingMainDriver.cpp:132-220: gather, then
+// process, once per time moment, until the stop flag), on inputs read from a
+// directory, and writes what the loop leaves behind.
+//
+//   shim_check <dir>
+// Inputs in <dir> (raw little-endian arrays):
+//   nodes.f64 (3M)  tets.i32 (4T)  tris.i32 (3S)  phi.f64 (M)
+//   pos.f64 (3N)  vel.f64 (3N)  type.i32 (N)  id.u64 (N)
+//   config.txt    one "key value" per line: dt, steps, model, gas, pressure,
+//                 temperature, seed, precision (fp64|mixed), density
+//                 (barycentric|tet_average), em_push (0|1), diameter_<Type>,
+//                 charge_<Type> (in units of e)
+// Outputs in <dir>:
+//   out_pos.f64 out_vel.f64  particle state after the loop (ParticleVector order)
+//   out_settled.u64           settled ids, ascending
+//   out_tri_counts.u64        TriangleCell::count by triangle tag (row order)
+//   out_density.f64 out_density_written.i32   the last gather's map, by node row
+//   out_summary.txt           steps run, stop flag
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+#include "StartProcessors.hpp"
+
+using namespace start_amd;
+
+template <typename T>
+static std::vector<T> read_array(const std::string& path)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot read " + path);
+    f.seekg(0, std::ios::end);
+    size_t bytes = (size_t)f.tellg();
+    f.seekg(0);
+    std::vector<T> v(bytes / sizeof(T));
+    f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
+    return v;
+}
+
+template <typename T>
+static void write_array(const std::string& path, const std::vector<T>& v)
+{
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
+    if (!f) throw std::runtime_error("cannot write " + path);
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 2) {
+        std::cerr << "usage: shim_check <dir>" << std::endl;
+        return 2;
+    }
+    std::string dir = argv[1];
+    try {
+        std::map<std::string, std::string> cfg;
+        {
+            std::ifstream f(dir + "/config.txt");
+            std::string k, v;
+            while (f >> k >> v) cfg[k] = v;
+        }
+        auto num = [&](const std::string& k) { return std::stod(cfg.at(k)); };
+        DeviceMeshInput mesh;
+        mesh.nodes = read_array<double>(dir + "/nodes.f64");
+        mesh.tets = read_array<int32_t>(dir + "/tets.i32");
+        mesh.triangles = read_array<int32_t>(dir + "/tris.i32");
+        DeviceKernelOptions opt;
+        opt.seed = (uint64_t)std::stoull(cfg.at("seed"));
+        opt.precision = cfg.at("precision") == "mixed" ? START_PREC_MIXED : START_PREC_FP64;
+        opt.densityMode = cfg.at("density") == "tet_average" ? START_DENSITY_TET_AVERAGE : START_DENSITY_BARYCENTRIC;
+        opt.emPush = cfg.count("em_push") ? std::stoi(cfg.at("em_push")) != 0 : true;
+        opt.writeBackEachStep = false;
+        for (const auto& [k, v] : cfg) {
+            if (k.rfind("diameter_", 0) == 0) opt.diameterOverride[typeFromName(k.substr(9))] = std::stod(v);
+            if (k.rfind("charge_", 0) == 0) opt.chargeOverride[typeFromName(k.substr(7))] = std::stod(v) * 1.602176565e-19;
+        }
+        size_t M = mesh.nodes.size() / 3;
+        ParticleKernelDevice device(mesh, opt);
+        std::vector<double> phi = read_array<double>(dir + "/phi.f64");
+        std::map<size_t, double> phiByTag;
+        for (size_t n = 0; n < M; ++n) phiByTag[n + 1] = phi[n];
+        device.setNodePotential(phiByTag);
+
+        std::vector<double> pos = read_array<double>(dir + "/pos.f64"), vel = read_array<double>(dir + "/vel.f64");
+        std::vector<int32_t> type = read_array<int32_t>(dir + "/type.i32");
+        std::vector<uint64_t> id = read_array<uint64_t>(dir + "/id.u64");
+        ParticleVector particles;
+        particles.reserve(type.size());
+        for (size_t i = 0; i < type.size(); ++i)
+            particles.emplace_back((size_t)id[i], (ParticleType)type[i], pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
+                                   vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+
+        const double dt = num("dt");
+        const int steps = std::stoi(cfg.at("steps"));
+        const double conc = start_gas_concentration(num("pressure"), num("temperature"));
+        ParticlesIDSet settled;
+        ParticleTrackerMap tracker;
+        ParticleMovementMap movements;
+        NodeChargeDensitiesMap density;
+        StopSubject stop;
+        std::shared_mutex sh;
+        std::mutex mx;
+        int k = 0;
+        double t = 0.0;
+        for (; k < steps && !stop.stopRequested(); ++k, t += dt) {
+            NodeChargeDensityProcessor::gather(t, dir + "/config.txt", device, particles, settled, tracker, density);
+            ParticleDynamicsProcessor::process(dir + "/config.txt", particles, t, dt, 16, device, tracker, settled,
+                                               sh, mx, movements, stop, cfg.at("model"), cfg.at("gas"), conc);
+        }
+        density.clear();
+        NodeChargeDensityProcessor::gather(t, dir + "/config.txt", device, particles, settled, tracker, density);
+        device.syncParticles(particles);
+
+        std::vector<double> op, ov;
+        for (const auto& p : particles) {
+            auto c = p.getCentre();
+            auto v = p.getVelocity();
+            op.insert(op.end(), c.begin(), c.end());
+            ov.insert(ov.end(), v.begin(), v.end());
+        }
+        write_array(dir + "/out_pos.f64", op);
+        write_array(dir + "/out_vel.f64", ov);
+        write_array(dir + "/out_settled.u64", std::vector<uint64_t>(settled.begin(), settled.end()));
+        std::vector<uint64_t> counts;
+        for (const auto& [tag, c] : device.triangleCounts()) counts.push_back(c);
+        write_array(dir + "/out_tri_counts.u64", counts);
+        std::vector<double> rho(M, 0.0);
+        std::vector<int32_t> wr(M, 0);
+        for (const auto& [tag, v] : density) {
+            rho[tag - 1] = v;
+            wr[tag - 1] = 1;
+        }
+        write_array(dir + "/out_density.f64", rho);
+        write_array(dir + "/out_density_written.i32", wr);
+        std::ofstream(dir + "/out_summary.txt") << "steps " << k << "\nstop " << (stop.stopRequested() ? 1 : 0) << "\n";
+        std::cout << "shim_check: " << k << " steps, " << settled.size() << " settled" << std::endl;
+    } catch (std::exception const& ex) {
+        std::cerr << "shim_check: " << ex.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

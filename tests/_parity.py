@@ -96,5 +96,21 @@ class Pair:
         log(f"{where}: ok {res}")
         return res
 
+    def compare_density(self, where=""):
+        """Both node charge density modes against the oracle at the current state."""
+        vn = self.om.node_volumes()
+        floor = E_CHARGE / vn
+        rho, wr = self.eng.node_charge_density("barycentric")
+        assert wr.all()
+        err = np.abs(rho - self.qo / vn) / np.maximum(np.abs(self.qo / vn), floor)
+        assert err.max() <= REL_Q, f"{where}: barycentric density rel error {err.max():.3e}"
+        rho_o, wr_o = self.sim.node_density(self.st)
+        rho, wr = self.eng.node_charge_density("tet_average")
+        assert np.array_equal(wr, wr_o), f"{where}: written nodes differ ({int((wr != wr_o).sum())})"
+        err2 = np.abs(rho - rho_o) / np.maximum(np.abs(rho_o), floor)
+        assert err2.max() <= REL_Q, f"{where}: tet-average density rel error {err2.max():.3e}"
+        log(f"{where}: density ok (bary {err.max():.2e}, tet-average {err2.max():.2e}, "
+            f"{int(wr.sum())} nodes written)")
+
     def close(self):
         self.eng.close()

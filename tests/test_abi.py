@@ -47,5 +47,20 @@ def test_bad_arguments_are_reported_not_thrown():
     assert L.start_read_stats(None, None) == -1
 
 
+def test_cpp_entry_points_built_and_exported():
+    """libstart_host.so exports ParticleDynamicsProcessor::process and
+    NodeChargeDensityProcessor::gather; shim_check links and runs its argument
+    check without touching a GPU."""
+    import subprocess
+    lib = os.path.join(ROOT, "start_amd", "libstart_host.so")
+    exe = os.path.join(ROOT, "start_amd", "shim_check")
+    assert os.path.exists(lib) and os.path.exists(exe), "run start_amd/build.py"
+    syms = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True).stdout
+    assert "start_amd::ParticleDynamicsProcessor::process(" in syms
+    assert "start_amd::NodeChargeDensityProcessor::gather(" in syms
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
 def test_oracle_library_loads(oracle_lib):
     assert oracle_lib.gas_concentration(1.0, 300.0) > 0

@@ -68,6 +68,32 @@ def test_small_box_charged_push():
     pr.close()
 
 
+def test_node_charge_density_modes():
+    """NodeChargeDensitiesMap values: barycentric Q_n / V_n and the reference's
+    tet-average formula, FP64 and mixed state."""
+    w = workloads.small_box(n_particles=3000, charge_units=2.0)
+    for prec in ("fp64", "mixed"):
+        pr = Pair(w, precision=prec)
+        pr.advance(40)
+        if prec == "fp64":
+            pr.compare(f"density {prec} step {pr.step}")
+            pr.compare_density(f"density {prec} step {pr.step}")
+        else:
+            # mixed: owners may differ from FP64 for a few particles; the
+            # tet-average formula is checked on the device's own owners
+            g = pr.eng.particles()
+            st = pr.st.copy()
+            st.tet[:] = g["tet"]
+            st.status[:] = g["status"]
+            rho_o, wr_o = pr.sim.node_density(st)
+            rho, wr = pr.eng.node_charge_density("tet_average")
+            assert np.array_equal(wr, wr_o)
+            floor = 1.602176565e-19 / pr.om.node_volumes()
+            err = np.abs(rho - rho_o) / np.maximum(np.abs(rho_o), floor)
+            assert err.max() <= 1e-12, err.max()
+        pr.close()
+
+
 def test_small_box_literal_reference_no_push():
     """em_push=False reproduces the reference main loop, where the owner lookup
     never matches and no particle is pushed."""

@@ -98,15 +98,74 @@ def gpu_run(w, mode: str, steps: int, warmup: int):
     }
 
 
+def sweep_run(steps: int, warmup: int, trial_steps: int = 200):
+    """The batched sweep: 64 runs of uneven size on the large box mesh in the
+    same launches (mixed precision), launch shape from the autotuner, steps in
+    captured hipGraphs; then the same sweep in FP64 for the deposition
+    histogram check."""
+    from start_amd import workloads
+    from start_amd.engine import ParticleInCellStep
+    t = time.time()
+    sw = workloads.sweep_box()
+    n = len(sw.pos)
+    log(f"[bench] sweep: {len(sw.runs)} runs, {n} particles, built in {time.time() - t:.1f}s")
+    out = {}
+    hists = {}
+    for prec in ("mixed", "fp64"):
+        t = time.time()
+        eng = ParticleInCellStep(sw.mesh, sw.species, sw.gas, sw.dt, sw.seed, precision=prec,
+                                 steps_per_launch=100, sort_interval=200, graph=True)
+        eng.set_field_basis(0, phi=sw.phi_unit)
+        eng.load_batch(sw.runs, sw.pos, sw.vel, sw.sp_idx)
+        eng.synchronize()
+        log(f"[bench] sweep {prec}: device set-up {time.time() - t:.1f}s")
+        if prec == "mixed":
+            t = time.time()
+            cfg, ms = eng.autotune(trial_steps=trial_steps)
+            eng.set_launch(**{**cfg, "graph": True})
+            log(f"[bench] sweep autotune ({time.time() - t:.1f}s): {cfg}, {ms:.4f} ms/step in trials")
+            out["autotuned"] = cfg
+        else:
+            eng.set_launch(**{**out["autotuned"], "graph": True})
+        eng.process(warmup)
+        eng.synchronize()
+        s0 = eng.stats()
+        t0 = time.perf_counter()
+        eng.process(steps)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        s1 = eng.stats()
+        hists[prec] = eng.tri_counts_runs().astype(np.float64)
+        K = eng.get_launch()["steps_per_launch"]
+        eng.timing(True)  # kernel time of step_main, outside the timed region
+        eng.process(max(K, 2 * K))
+        eng.synchronize()
+        kern_ms, launches = eng.timing_read()
+        s2 = eng.stats()
+        eng.close()
+        processed = s1["processed"] - s0["processed"]
+        out[prec] = {"processed": processed, "wall_s": t1 - t0, "ps_per_s": processed / (t1 - t0),
+                     "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms_per_launch": kern_ms / max(launches, 1),
+                     "processed_per_launch": (s2["processed"] - s1["processed"]) / max(launches, 1),
+                     "steps_per_launch": K}
+        log(f"[bench] sweep {prec}: {out[prec]}")
+    h64 = hists["fp64"]
+    out["deposition_l1_vs_fp64"] = float(np.abs(hists["mixed"] - h64).sum() / max(h64.sum(), 1.0))
+    out["deposited"] = int(h64.sum())
+    out["runs"] = len(sw.runs)
+    out["particles"] = n
+    return out
+
+
 def roofline(r, mode):
-    prec = MODES[mode][0]
+    prec = MODES[mode][0] if mode in MODES else r["precision"]
     B = BYTES_PER_PS[prec]
     dur = r["kernel_ms_per_launch"] * 1e-3
     achieved = B * r["processed_per_launch"] / dur / 1e9 if dur > 0 else 0.0
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(mode),
             "kernel": f"step_main<{'double' if prec == 'fp64' else 'float'}>",
-            "bytes_per_particle_step": B, "steps_per_launch": MODES[mode][1],
+            "bytes_per_particle_step": B, "steps_per_launch": r["steps_per_launch"],
             "particle_steps_per_launch": round(r["processed_per_launch"]),
             "launch_ms": round(r["kernel_ms_per_launch"], 4)}
 
@@ -158,6 +217,8 @@ def main():
     ap.add_argument("--mode-steps", type=int, default=400, help="timed steps of the non-headline modes")
     ap.add_argument("--particles", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--sweep-steps", type=int, default=500,
+                    help="timed steps of the batched sweep (64 runs); 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES, 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -188,6 +249,9 @@ def main():
         steps = args.steps if m == HEADLINE else args.mode_steps
         results[m] = gpu_run(w, m, steps, args.warmup)
         log(f"[bench] {m}: {results[m]}")
+    sweep = None
+    if args.sweep_steps > 0:
+        sweep = sweep_run(args.sweep_steps, args.warmup)
     head = HEADLINE if HEADLINE in results else modes[0]
     r = results[head]
 
@@ -223,6 +287,23 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu_baseline"] = round(r["ps_per_s"] / cpu["value"], 1)
+    if sweep:
+        m, f = sweep["mixed"], sweep["fp64"]
+        m["precision"], f["precision"] = "mixed", "fp64"
+        out["sweep"] = {
+            "workload": "sweep_box: 64 runs on the 750,000-tet box, counts log-uniform in [1e4, 2e6] "
+                        f"({sweep['particles']} particles), 0-1000 V, 0.1-5 Pa Ar VSS, dt 1e-9 s",
+            "steps": args.sweep_steps, "warmup": args.warmup, "autotuned": sweep["autotuned"],
+            "mixed": {"value": round(m["ps_per_s"], 1), "ms_per_step": round(m["ms_per_step"], 5),
+                      "roofline": roofline(m, "mixed_sweep")},
+            "fp64": {"value": round(f["ps_per_s"], 1), "ms_per_step": round(f["ms_per_step"], 5),
+                     "roofline": roofline(f, "fp64_sweep")},
+            "deposited": sweep["deposited"],
+            "deposition_l1_vs_fp64": sweep["deposition_l1_vs_fp64"],
+            "mixed_vs_headline": round(m["ps_per_s"] / r["ps_per_s"], 3),
+        }
+        if "fp64_per_step" in results:
+            out["sweep"]["mixed_vs_fp64_per_step"] = round(m["ps_per_s"] / results["fp64_per_step"]["ps_per_s"], 3)
     print(json.dumps(out), flush=True)
 
 

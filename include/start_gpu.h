@@ -128,6 +128,48 @@ int start_load_particles(start_ctx* ctx, int64_t n, const double* x, const doubl
                          const double* z, const double* vx, const double* vy, const double* vz,
                          const int32_t* species, const uint64_t* ids);
 
+/* ---- batched sweeps: independent runs on one mesh in the same launches ----
+ * (no counterpart in the reference, which runs one simulation per process;
+ * each run behaves exactly like a context of its own loaded with that run's
+ * particles, field, concentration and variate key). */
+typedef struct start_run_desc {
+    int64_t n_particles;   /* rows of this run: the runs' rows follow each other in run order */
+    double field_scale[2]; /* E of the run = s0 * E_0 + s1 * E_1 (field bases 0 and 1)       */
+    double concentration;  /* background gas concentration of the run, m^-3                  */
+    uint32_t run;          /* folded into the variate key                                     */
+    int32_t pad;
+} start_run_desc;
+
+/* Uploads the particles of n_runs runs (sum of n_particles rows) and locates
+ * them; replaces any particles loaded before.  Every run keeps its own node
+ * charge and triangle counters. */
+int start_load_batch(start_ctx* ctx, int32_t n_runs, const start_run_desc* runs, const double* x,
+                     const double* y, const double* z, const double* vx, const double* vy,
+                     const double* vz, const int32_t* species, const uint64_t* ids);
+/* Field basis `which` (0 or 1): per-tet E or node potentials (E = -grad phi).
+ * Basis 0 is the field start_set_tet_field / start_set_node_potential set. */
+int start_set_tet_field_basis(start_ctx* ctx, int32_t which, const double* E);
+int start_set_node_potential_basis(start_ctx* ctx, int32_t which, const double* phi);
+int start_read_node_charge_runs(start_ctx* ctx, double* out /* n_runs*n_nodes, run-major */);
+int start_read_tri_counts_runs(start_ctx* ctx, uint64_t* out /* n_runs*n_tris, run-major */);
+
+/* ---- launch configuration and the autotuner ---- */
+typedef struct start_launch_cfg {
+    int32_t steps_per_launch; /* steps fused per launch, >= 1                                */
+    int32_t block;            /* threads per workgroup: 128 or 256                            */
+    int32_t persistent;       /* 1: a grid sized to the chip pulls particle tiles from a queue */
+    int32_t sort_interval;    /* steps between sorts by tet; 0 = never                        */
+    int32_t graph;            /* 1: the launches between two sorts are captured once in a
+                                 hipGraph and replayed                                        */
+    int32_t pad;
+} start_launch_cfg;
+int start_set_launch(start_ctx* ctx, const start_launch_cfg* cfg);
+int start_get_launch(start_ctx* ctx, start_launch_cfg* cfg);
+/* Times each candidate configuration (steps per launch, block, persistent,
+ * sort interval) for trial_steps steps from a copy of the current state,
+ * restores the state, and sets the fastest.  best and ms_per_step may be NULL. */
+int start_autotune(start_ctx* ctx, int32_t trial_steps, start_launch_cfg* best, double* ms_per_step);
+
 /* Replaces the background gas (model, target, concentration) and, when
  * diameters is not NULL, the species' collision diameters (n_species values,
  * r_p + r_t depends on the target); recomputes the collision constants.

@@ -24,6 +24,9 @@ EXPORTS = [
     "start_read_tri_counts", "start_read_settled", "start_read_stats", "start_synchronize",
     "start_locate", "start_orient3d_batch", "start_timing_enable", "start_timing_read",
     "start_node_charge_density", "start_set_gas", "start_set_time_step",
+    "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
+    "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
+    "start_autotune",
 ]
 DENSITY = {"barycentric": 0, "tet_average": 1}
 
@@ -55,6 +58,16 @@ class Stats(C.Structure):
     _fields_ = [("step", C.c_uint64), ("processed", C.c_uint64), ("settled", C.c_uint64),
                 ("n_particles", C.c_uint64), ("deferred", C.c_uint64),
                 ("exhaustive", C.c_uint64), ("stop", C.c_int32), ("pad", C.c_int32)]
+
+
+class RunDesc(C.Structure):
+    _fields_ = [("n_particles", C.c_int64), ("field_scale", C.c_double * 2), ("concentration", C.c_double),
+                ("run", C.c_uint32), ("pad", C.c_int32)]
+
+
+class LaunchCfg(C.Structure):
+    _fields_ = [("steps_per_launch", C.c_int32), ("block", C.c_int32), ("persistent", C.c_int32),
+                ("sort_interval", C.c_int32), ("graph", C.c_int32), ("pad", C.c_int32)]
 
 
 _LIB = None
@@ -95,6 +108,14 @@ def lib() -> C.CDLL:
     L.start_node_charge_density.argtypes = [vp, i32, P(d), P(i32)]
     L.start_set_gas.argtypes = [vp, P(GasDesc), P(d)]
     L.start_set_time_step.argtypes = [vp, d]
+    L.start_load_batch.argtypes = [vp, i32, P(RunDesc)] + [P(d)] * 6 + [P(i32), P(u64)]
+    L.start_set_tet_field_basis.argtypes = [vp, i32, P(d)]
+    L.start_set_node_potential_basis.argtypes = [vp, i32, P(d)]
+    L.start_read_node_charge_runs.argtypes = [vp, P(d)]
+    L.start_read_tri_counts_runs.argtypes = [vp, P(u64)]
+    L.start_set_launch.argtypes = [vp, P(LaunchCfg)]
+    L.start_get_launch.argtypes = [vp, P(LaunchCfg)]
+    L.start_autotune.argtypes = [vp, i32, P(LaunchCfg), P(d)]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is C.c_int:  # default

@@ -4,7 +4,7 @@
 //   x y z vx vy vz   FP64 (FP64 mode) or FP32 (mixed mode)
 //   tet              int32  owning tet at the start of the next step (-1: none)
 //   flags            int32  bit0 settled, bit1 position strictly inside `tet`,
-//                           bits 8..15 species index
+//                           bits 8..15 species index, bits 16..31 run index
 //   hit              int32  surface triangle the particle settled on (-1: none)
 //   pidx             uint32 load index (variate counter)
 // Tets: geo (12 doubles: v0, 3x3 barycentric map), E (3 doubles),
@@ -23,6 +23,8 @@ constexpr int32_t kSettled = 1;
 constexpr int32_t kStrict = 2;
 constexpr int kSpeciesShift = 8;
 constexpr int kMaxSpecies = 32;
+constexpr int kRunShift = 16;
+constexpr int kMaxRuns = 1 << 15;
 
 // device counters (unsigned long long)
 enum Counter {
@@ -33,9 +35,23 @@ enum Counter {
     kCntWork = 4,   // work-list length, slot 0 (slot 1 at kCntWork + 2)
     kCntError = 5,  // bit 0: expansion capacity
     kCntProcSlot0 = 8,  // processed particle-steps, spread over kProcSlots slots
-    kCntTotal = 8 + 64
+    kCntStep = 72,      // first step of the launch, slot 0 (slot 1 at kCntStep + 1)
+    kCntTile = 74,      // tile queue of the persistent launch, slot 0 (slot 1 at kCntTile + 1)
+    kCntTotal = 80
 };
 constexpr int kProcSlots = 64;
+
+// One independent run of a batch: its gas concentration, its field as a
+// combination of the two field bases, its variate key and where its particles,
+// node charge and triangle counters start.
+struct RunK {
+    double conc;
+    double s0, s1;       // E = s0 * E_0 + s1 * E_1
+    uint32_t key;        // folded into the variate key
+    uint32_t first;      // load index of the run's first particle
+    int64_t node_base;   // run * n_nodes
+    int64_t tri_base;    // run * n_tris
+};
 
 struct WorkItem {
     double x0, y0, z0;  // absolute start of the deferred step
@@ -48,8 +64,10 @@ struct WorkItem {
 struct DevMesh {
     const double* nodes;   // 3M
     const double* geo;     // 12T
-    const double* E;       // 3T (FP64)
-    const float* E32;      // 4T (mixed)
+    const double* E;       // 3T (FP64), field basis 0
+    const float* E32;      // 4T (mixed), field basis 0
+    const double* E1;      // 3T, field basis 1 (null: one basis)
+    const float* E1_32;    // 4T
     const float* geo32;    // 12T (mixed)
     const int4* tnode;     // T
     const int4* tnbr;      // T
@@ -75,20 +93,27 @@ struct DevParticles {
     int64_t n;
 };
 
+// Shape of a step launch: workgroup size and tile scheduling.
+struct StepLaunch {
+    int block;       // 128 | 256 threads
+    int persistent;  // 1: `grid` workgroups pull tiles from a queue
+    int grid;
+    int batch;       // 1: particles of several runs (per-lane run parameters)
+};
+
 struct StepParams {
     double dt;
-    double conc;
     double lo[3], hi[3];
     uint64_t seed;
-    uint32_t run;
-    uint32_t step;
+    const RunK* runs;         // run table (one entry for a single simulation)
     int32_t model;
     int32_t em_push;
     int32_t charge;
     int32_t nspecies;
     const int64_t* n_active;  // live particles lead the arrays after a sort
-    double* zero_charge;      // node-charge buffer of the next step, cleared here
-    int32_t work_slot;        // work-list counter of this launch: cnt[kCntWork + 2*slot]
+    double* zero_charge;      // node-charge buffers of the next launch, cleared here
+    int64_t charge_len;       // n_runs * n_nodes
+    int32_t work_slot;        // slot of this launch's work-list, step and tile counters
     int32_t nsteps;           // steps fused in this launch (charge accumulates over them)
 };
 

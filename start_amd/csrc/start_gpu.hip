@@ -1,5 +1,6 @@
 // C ABI of libstart_gpu (declared in include/start_gpu.h): device state of one
-// simulation context and the host side of the step loop.
+// simulation context (or one batch of independent runs on a shared mesh) and
+// the host side of the step loop.
 #include "start_gpu.h"
 
 #include <math.h>
@@ -14,15 +15,15 @@
 #include "mesh_prep.h"
 
 namespace sg {
-void launch_zero_charge(hipStream_t s, double* a, double* b, int64_t n_nodes);
+void launch_zero_charge(hipStream_t s, double* a, double* b, int64_t n);
 template <typename R>
 void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const StepParams& S, const SpeciesK* sp,
                  double* node_charge, WorkItem* wl, unsigned long long* cnt, unsigned long long* tri_count,
-                 hipEvent_t ev_mid);
+                 const StepLaunch& L, hipEvent_t ev_mid);
 void launch_locate(hipStream_t s, const DevMesh& M, const double* x, const double* y, const double* z, int64_t n,
                    int32_t* tet, int32_t* flags, float* lx, float* ly, float* lz, unsigned long long* cnt);
 template <typename R>
-void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,
+void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp, const RunK* runs,
                    double* node_charge);
 template <typename R>
 void launch_node_density(hipStream_t s, const DevParticles<R>& P, const SpeciesK* sp, int32_t mode,
@@ -59,6 +60,23 @@ struct PBuf {
     uint32_t* pidx = nullptr;
 };
 
+// A captured chunk of step launches: valid for the live buffer set, the
+// charge/work-list parities and the launch configuration it was captured with.
+struct GraphEntry {
+    const void* ax;
+    int cb, wslot;
+    int32_t chunk;
+    start_launch_cfg cfg;
+    hipGraphExec_t exec;
+    int cb_after, wslot_after, latest_after;
+};
+
+bool same_cfg(const start_launch_cfg& a, const start_launch_cfg& b)
+{
+    return a.steps_per_launch == b.steps_per_launch && a.block == b.block && a.persistent == b.persistent &&
+           a.sort_interval == b.sort_interval;
+}
+
 }  // namespace
 
 struct start_ctx {
@@ -66,6 +84,8 @@ struct start_ctx {
     hipStream_t stream = nullptr;
     HostMesh hm;
     start_params prm{};
+    start_launch_cfg cfg{};
+    int n_cu = 256;
     std::vector<start_species_desc> species_desc;
     bool mixed = false;
     size_t es = sizeof(double);  // bytes of one position/velocity component
@@ -76,18 +96,23 @@ struct start_ctx {
     float* d_geo32 = nullptr;
     double* d_E = nullptr;
     float* d_E32 = nullptr;
+    double* d_E1 = nullptr;  // field basis 1 (allocated on first use)
+    float* d_E1_32 = nullptr;
     int4* d_tnode = nullptr;
     int4* d_tnbr = nullptr;
     int32_t* d_tris = nullptr;
     uint8_t* d_tri_deg = nullptr;
     int32_t *d_tgs = nullptr, *d_tgl = nullptr, *d_sgs = nullptr, *d_sgl = nullptr;
-    unsigned long long* d_tri_count = nullptr;
-    double* d_node_charge = nullptr;  // two buffers of n_nodes: this launch's and the next one's
-    int cb = 0;                       // buffer the next launch scatters into
-    int latest = 0;                   // buffer holding the most recent node charge
-    int wslot = 0;                    // work-list counter slot of the next launch
     SpeciesK* d_species = nullptr;
     float tau32 = 0.0f;
+    // runs: one for a single simulation, n_runs for a batch
+    std::vector<RunK> runs;
+    RunK* d_runs = nullptr;
+    unsigned long long* d_tri_count = nullptr;  // n_runs * n_tris
+    double* d_node_charge = nullptr;            // two buffers of n_runs * n_nodes
+    int cb = 0;                                 // buffer the next launch scatters into
+    int latest = 0;                             // buffer holding the most recent node charge
+    int wslot = 0;                              // work-list / step / tile counter slot of the next launch
     // node charge density tables (built on first use)
     double *d_tet_vol = nullptr, *d_node_vol = nullptr, *d_tet_q = nullptr;
     int32_t *d_adj_start = nullptr, *d_adj_list = nullptr, *d_tet_n = nullptr;
@@ -104,10 +129,15 @@ struct start_ctx {
     unsigned long long* d_cnt = nullptr;
     uint64_t step = 0;
     uint64_t last_sort = 0;
+    // captured step chunks
+    std::vector<GraphEntry> graphs;
     // timing
     bool timing = false;
     std::vector<TimedStep> events;
     size_t ev_used = 0;
+
+    int64_t n_runs() const { return (int64_t)runs.size(); }
+    int64_t charge_len() const { return n_runs() * hm.n_nodes; }
 
     DevMesh mesh_view() const
     {
@@ -116,6 +146,8 @@ struct start_ctx {
         M.geo = d_geo;
         M.E = d_E;
         M.E32 = d_E32;
+        M.E1 = d_E1;
+        M.E1_32 = d_E1_32;
         M.geo32 = d_geo32;
         M.tnode = d_tnode;
         M.tnbr = d_tnbr;
@@ -138,6 +170,15 @@ struct start_ctx {
     {
         return DevParticles<R>{(R*)A.x, (R*)A.y, (R*)A.z, (R*)A.vx, (R*)A.vy, (R*)A.vz,
                                A.tet,   A.flags, A.hit,  A.pidx,  n};
+    }
+    StepLaunch launch_shape() const
+    {
+        StepLaunch L{};
+        L.block = cfg.block == 128 ? 128 : 256;
+        L.persistent = cfg.persistent ? 1 : 0;
+        L.grid = n_cu * (1024 / L.block);  // 4 waves per SIMD x 4 SIMDs per CU
+        L.batch = runs.size() > 1 ? 1 : 0;
+        return L;
     }
 };
 
@@ -186,8 +227,15 @@ void free_pbuf(PBuf& b)
     b = PBuf{};
 }
 
+void drop_graphs(start_ctx* c)
+{
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
 void free_particles(start_ctx* c)
 {
+    drop_graphs(c);
     free_pbuf(c->A);
     free_pbuf(c->B);
     void* ptrs[] = {c->d_wl, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp};
@@ -345,6 +393,231 @@ int check_kernel_errors(start_ctx* c)
     return START_OK;
 }
 
+int upload_runs(start_ctx* c)
+{
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (c->d_runs) (void)hipFree(c->d_runs);
+    c->d_runs = nullptr;
+    HIPCHECK(c, dalloc(&c->d_runs, c->runs.size()));
+    HIPCHECK(c, hipMemcpy(c->d_runs, c->runs.data(), sizeof(RunK) * c->runs.size(), hipMemcpyHostToDevice));
+    drop_graphs(c);  // captured launches hold the old table pointer
+    return START_OK;
+}
+
+// Per-run output buffers: node charge (two sets) and triangle counters.
+int alloc_run_outputs(start_ctx* c)
+{
+    if (c->d_node_charge) (void)hipFree(c->d_node_charge);
+    if (c->d_tri_count) (void)hipFree(c->d_tri_count);
+    c->d_node_charge = nullptr;
+    c->d_tri_count = nullptr;
+    const int64_t R = c->n_runs();
+    hipError_t e = dalloc(&c->d_node_charge, 2 * R * c->hm.n_nodes);
+    if (e == hipSuccess) e = dalloc(&c->d_tri_count, R * std::max<int64_t>(c->hm.n_tris, 1));
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("run buffers: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+// Fills the step arguments of the current configuration.
+StepParams step_params(const start_ctx* c)
+{
+    StepParams S{};
+    S.dt = c->prm.dt;
+    for (int k = 0; k < 3; ++k) {
+        S.lo[k] = c->hm.bbox_lo[k];
+        S.hi[k] = c->hm.bbox_hi[k];
+    }
+    S.seed = c->prm.seed;
+    S.runs = c->d_runs;
+    S.model = c->prm.gas.model;
+    S.em_push = c->prm.em_push;
+    S.charge = c->prm.charge_every_step;
+    S.nspecies = (int32_t)c->species.size();
+    S.n_active = c->d_nactive;
+    S.charge_len = c->charge_len();
+    return S;
+}
+
+// Enqueues `chunk` steps as launches of at most steps_per_launch steps and
+// advances the host-side parities.  No sort happens inside a chunk.
+int enqueue_chunk(start_ctx* c, int32_t chunk)
+{
+    DevMesh M = c->mesh_view();
+    StepParams S = step_params(c);
+    const StepLaunch L = c->launch_shape();
+    const int64_t CL = c->charge_len();
+    const int32_t K = std::max(1, c->cfg.steps_per_launch);
+    int32_t done = 0;
+    while (done < chunk) {
+        int32_t kk = std::min(K, chunk - done);
+        double* nc = c->prm.charge_every_step ? c->d_node_charge + (size_t)c->cb * CL : nullptr;
+        S.zero_charge = c->prm.charge_every_step ? c->d_node_charge + (size_t)(1 - c->cb) * CL : nullptr;
+        S.work_slot = c->wslot;
+        S.nsteps = kk;
+        TimedStep* ev = nullptr;
+        if (c->timing) {
+            if (c->ev_used == c->events.size()) {
+                TimedStep t{};
+                HIPCHECK(c, hipEventCreate(&t.a0));
+                HIPCHECK(c, hipEventCreate(&t.a1));
+                HIPCHECK(c, hipEventCreate(&t.b1));
+                c->events.push_back(t);
+            }
+            ev = &c->events[c->ev_used++];
+        }
+        if (ev) HIPCHECK(c, hipEventRecord(ev->a0, c->stream));
+        if (c->mixed)
+            launch_step<float>(c->stream, M, c->view<float>(), S, c->d_species, nc, c->d_wl, c->d_cnt, c->d_tri_count,
+                               L, ev ? ev->a1 : nullptr);
+        else
+            launch_step<double>(c->stream, M, c->view<double>(), S, c->d_species, nc, c->d_wl, c->d_cnt,
+                                c->d_tri_count, L, ev ? ev->a1 : nullptr);
+        if (ev) HIPCHECK(c, hipEventRecord(ev->b1, c->stream));
+        HIPCHECK(c, hipGetLastError());
+        done += kk;
+        c->wslot ^= 1;
+        if (nc) {
+            c->latest = c->cb;
+            c->cb ^= 1;
+        }
+    }
+    return START_OK;
+}
+
+// A chunk through a captured graph: replays the one captured for this state
+// and configuration, capturing it first if there is none.
+int run_chunk_graph(start_ctx* c, int32_t chunk)
+{
+    for (auto& g : c->graphs) {
+        if (g.ax == c->A.x && g.cb == c->cb && g.wslot == c->wslot && g.chunk == chunk && same_cfg(g.cfg, c->cfg)) {
+            HIPCHECK(c, hipGraphLaunch(g.exec, c->stream));
+            c->cb = g.cb_after;
+            c->wslot = g.wslot_after;
+            c->latest = g.latest_after;
+            return START_OK;
+        }
+    }
+    if (c->graphs.size() >= 16) drop_graphs(c);
+    GraphEntry g{};
+    g.ax = c->A.x;
+    g.cb = c->cb;
+    g.wslot = c->wslot;
+    g.chunk = chunk;
+    g.cfg = c->cfg;
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_chunk(c, chunk);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc != START_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    g.cb_after = c->cb;
+    g.wslot_after = c->wslot;
+    g.latest_after = c->latest;
+    c->graphs.push_back(g);
+    HIPCHECK(c, hipGraphLaunch(g.exec, c->stream));  // the capture only recorded the launches
+    return START_OK;
+}
+
+// Copies of the mutable device state, for the autotuner's trials.
+struct Snapshot {
+    PBuf P;
+    unsigned long long* cnt = nullptr;
+    unsigned long long* tri = nullptr;
+    double* charge = nullptr;
+    int64_t* nactive = nullptr;
+    int cb = 0, latest = 0, wslot = 0;
+    uint64_t step = 0, last_sort = 0;
+};
+
+hipError_t copy_pbuf(const PBuf& dst, const PBuf& src, int64_t n, size_t es, hipStream_t s)
+{
+    const void* from[10] = {src.x, src.y, src.z, src.vx, src.vy, src.vz, src.tet, src.flags, src.hit, src.pidx};
+    void* to[10] = {dst.x, dst.y, dst.z, dst.vx, dst.vy, dst.vz, dst.tet, dst.flags, dst.hit, dst.pidx};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 10 && e == hipSuccess; ++k)
+        e = hipMemcpyAsync(to[k], from[k], (k < 6 ? es : 4) * (size_t)n, hipMemcpyDeviceToDevice, s);
+    return e;
+}
+
+int snapshot_take(start_ctx* c, Snapshot& s)
+{
+    const int64_t R = c->n_runs();
+    hipError_t e = alloc_pbuf(s.P, c->n, c->es);
+    if (e == hipSuccess) e = dalloc(&s.cnt, kCntTotal);
+    if (e == hipSuccess) e = dalloc(&s.tri, R * std::max<int64_t>(c->hm.n_tris, 1));
+    if (e == hipSuccess) e = dalloc(&s.charge, 2 * R * c->hm.n_nodes);
+    if (e == hipSuccess) e = dalloc(&s.nactive, 1);
+    if (e == hipSuccess) e = copy_pbuf(s.P, c->A, c->n, c->es, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.cnt, c->d_cnt, sizeof(unsigned long long) * kCntTotal, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.tri, c->d_tri_count, sizeof(unsigned long long) * R * std::max<int64_t>(c->hm.n_tris, 1),
+                           hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.charge, c->d_node_charge, sizeof(double) * 2 * R * c->hm.n_nodes,
+                           hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.nactive, c->d_nactive, sizeof(int64_t), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    s.cb = c->cb;
+    s.latest = c->latest;
+    s.wslot = c->wslot;
+    s.step = c->step;
+    s.last_sort = c->last_sort;
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("state snapshot: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+int snapshot_restore(start_ctx* c, const Snapshot& s)
+{
+    const int64_t R = c->n_runs();
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    hipError_t e = copy_pbuf(c->A, s.P, c->n, c->es, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_cnt, s.cnt, sizeof(unsigned long long) * kCntTotal, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_tri_count, s.tri, sizeof(unsigned long long) * R * std::max<int64_t>(c->hm.n_tris, 1),
+                           hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_node_charge, s.charge, sizeof(double) * 2 * R * c->hm.n_nodes,
+                           hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_nactive, s.nactive, sizeof(int64_t), hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    c->cb = s.cb;
+    c->latest = s.latest;
+    c->wslot = s.wslot;
+    c->step = s.step;
+    c->last_sort = s.last_sort;
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("state restore: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+void snapshot_free(Snapshot& s)
+{
+    free_pbuf(s.P);
+    void* ptrs[] = {s.cnt, s.tri, s.charge, s.nactive};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    s = Snapshot{};
+}
+
+int check_cfg(start_ctx* c, const start_launch_cfg& g)
+{
+    if (g.steps_per_launch < 1) return fail(c, START_ERR_ARG, "steps_per_launch must be >= 1");
+    if (g.block != 128 && g.block != 256) return fail(c, START_ERR_ARG, "block must be 128 or 256");
+    if (g.sort_interval < 0) return fail(c, START_ERR_ARG, "sort_interval must be >= 0");
+    return START_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -376,7 +649,17 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
     if (!c) return fail(nullptr, START_ERR_NOMEM, "host allocation failed");
     c->prm = *params;
     c->prm.species = nullptr;
-    if (c->prm.steps_per_launch < 1) c->prm.steps_per_launch = 1;
+    c->cfg.steps_per_launch = std::max(1, params->steps_per_launch);
+    c->cfg.block = 256;
+    c->cfg.persistent = 0;
+    c->cfg.sort_interval = params->sort_interval;
+    c->cfg.graph = 0;
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            c->n_cu = ncu;
+    }
     c->mixed = params->precision == START_PREC_MIXED;
     c->es = c->mixed ? sizeof(float) : sizeof(double);
     double expansion = params->bbox_expansion > 0.0 ? params->bbox_expansion : 1.01;
@@ -390,6 +673,12 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         c->species_desc.push_back(params->species[s]);
         c->species.push_back(species_constants(params->species[s], params->gas));
     }
+    RunK r0{};
+    r0.conc = params->gas.concentration;
+    r0.s0 = 1.0;
+    r0.s1 = 0.0;
+    r0.key = params->run;
+    c->runs.assign(1, r0);
     std::vector<float> g32;
     if (c->mixed) {
         c->tau32 = build_geo32(c->hm, g32);
@@ -429,11 +718,12 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
     chk(dalloc(&c->d_tgl, hm.tgrid_list.size()));
     chk(dalloc(&c->d_sgs, hm.sgrid_start.size()));
     chk(dalloc(&c->d_sgl, hm.sgrid_list.size()));
-    chk(dalloc(&c->d_tri_count, S));
-    chk(dalloc(&c->d_node_charge, 2 * M));
     chk(dalloc(&c->d_species, c->species.size()));
     chk(dalloc(&c->d_cnt, kCntTotal));
     chk(dalloc(&c->d_nactive, 1));
+    chk(dalloc(&c->d_runs, 1));
+    chk(dalloc(&c->d_node_charge, 2 * M));
+    chk(dalloc(&c->d_tri_count, std::max<int64_t>(S, 1)));
     if (he == hipSuccess) {
         chk(up(c->d_nodes, hm.nodes.data(), sizeof(double) * 3 * M));
         chk(up(c->d_geo, hm.geo.data(), sizeof(double) * 12 * T));
@@ -453,6 +743,7 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         chk(hipMemset(c->d_tri_count, 0, sizeof(unsigned long long) * std::max<int64_t>(S, 1)));
         chk(hipMemset(c->d_node_charge, 0, sizeof(double) * 2 * M));
         chk(up(c->d_species, c->species.data(), sizeof(SpeciesK) * c->species.size()));
+        chk(up(c->d_runs, c->runs.data(), sizeof(RunK)));
         chk(hipMemset(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal));
     }
     if (he != hipSuccess) {
@@ -469,10 +760,11 @@ void start_destroy(start_ctx* c)
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_particles(c);
-    void* ptrs[] = {c->d_nodes,     c->d_geo,      c->d_geo32,    c->d_E,         c->d_E32,     c->d_tnode,
-                    c->d_tnbr,      c->d_tris,     c->d_tri_deg,  c->d_tgs,       c->d_tgl,     c->d_sgs,
-                    c->d_sgl,       c->d_tri_count, c->d_node_charge, c->d_species, c->d_cnt,  c->d_nactive,
-                    c->d_tet_vol,   c->d_node_vol, c->d_tet_q,    c->d_adj_start, c->d_adj_list, c->d_tet_n};
+    void* ptrs[] = {c->d_nodes,     c->d_geo,       c->d_geo32,       c->d_E,         c->d_E32,      c->d_E1,
+                    c->d_E1_32,     c->d_tnode,     c->d_tnbr,        c->d_tris,      c->d_tri_deg,  c->d_tgs,
+                    c->d_tgl,       c->d_sgs,       c->d_sgl,         c->d_tri_count, c->d_node_charge,
+                    c->d_species,   c->d_cnt,       c->d_nactive,     c->d_runs,      c->d_tet_vol,  c->d_node_vol,
+                    c->d_tet_q,     c->d_adj_start, c->d_adj_list,    c->d_tet_n};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : c->events) {
@@ -484,34 +776,61 @@ void start_destroy(start_ctx* c)
     delete c;
 }
 
-int start_load_particles(start_ctx* c, int64_t n, const double* x, const double* y, const double* z,
-                         const double* vx, const double* vy, const double* vz, const int32_t* species,
-                         const uint64_t* ids)
+int start_load_batch(start_ctx* c, int32_t n_runs, const start_run_desc* runs, const double* x, const double* y,
+                     const double* z, const double* vx, const double* vy, const double* vz, const int32_t* species,
+                     const uint64_t* ids)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
-    if (n < 0 || n > (int64_t)0x7fffffff) return fail(c, START_ERR_ARG, "particle count out of range");
+    if (n_runs < 1 || n_runs > kMaxRuns || !runs) return fail(c, START_ERR_ARG, "n_runs must be in [1, 32768]");
+    int64_t n = 0;
+    for (int32_t r = 0; r < n_runs; ++r) {
+        if (runs[r].n_particles < 0) return fail(c, START_ERR_ARG, "negative run size");
+        if (!(runs[r].concentration >= 0.0)) return fail(c, START_ERR_ARG, "negative gas concentration");
+        n += runs[r].n_particles;
+    }
+    if (n > (int64_t)0x7fffffff) return fail(c, START_ERR_ARG, "particle count out of range");
+    if ((int64_t)n_runs * c->hm.n_nodes > (int64_t)0x7fffffff)
+        return fail(c, START_ERR_ARG, "n_runs * n_nodes exceeds the node-charge index range");
     if (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !species)) return fail(c, START_ERR_ARG, "null particle array");
     std::vector<int32_t> flags((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-        if (species[i] < 0 || species[i] >= (int32_t)c->species.size())
-            return fail(c, START_ERR_ARG, "species index out of range at particle " + std::to_string(i));
-        flags[(size_t)i] = species[i] << kSpeciesShift;
+    std::vector<RunK> rk((size_t)n_runs);
+    {
+        int64_t i = 0;
+        for (int32_t r = 0; r < n_runs; ++r) {
+            rk[(size_t)r].conc = runs[r].concentration;
+            rk[(size_t)r].s0 = runs[r].field_scale[0];
+            rk[(size_t)r].s1 = runs[r].field_scale[1];
+            rk[(size_t)r].key = runs[r].run;
+            rk[(size_t)r].first = (uint32_t)i;
+            rk[(size_t)r].node_base = (int64_t)r * c->hm.n_nodes;
+            rk[(size_t)r].tri_base = (int64_t)r * c->hm.n_tris;
+            for (int64_t j = 0; j < runs[r].n_particles; ++j, ++i) {
+                if (species[i] < 0 || species[i] >= (int32_t)c->species.size())
+                    return fail(c, START_ERR_ARG, "species index out of range at particle " + std::to_string(i));
+                flags[(size_t)i] = (species[i] << kSpeciesShift) | (r << kRunShift);
+            }
+        }
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     free_particles(c);
+    c->runs = rk;
+    int rc = upload_runs(c);
+    if (rc != START_OK) return rc;
+    rc = alloc_run_outputs(c);
+    if (rc != START_OK) return rc;
     hipError_t he = alloc_pbuf(c->A, n, c->es);
     if (he == hipSuccess) he = dalloc(&c->d_wl, n);
-    if (he == hipSuccess && c->prm.sort_interval > 0) {
-        he = alloc_pbuf(c->B, n, c->es);
-        if (he == hipSuccess) he = dalloc(&c->d_keys, n);
-        if (he == hipSuccess) he = dalloc(&c->d_keys_out, n);
-        if (he == hipSuccess) he = dalloc(&c->d_idx, n);
-        if (he == hipSuccess) he = dalloc(&c->d_perm, n);
+    if (he == hipSuccess) he = alloc_pbuf(c->B, n, c->es);
+    if (he == hipSuccess) he = dalloc(&c->d_keys, n);
+    if (he == hipSuccess) he = dalloc(&c->d_keys_out, n);
+    if (he == hipSuccess) he = dalloc(&c->d_idx, n);
+    if (he == hipSuccess) he = dalloc(&c->d_perm, n);
+    if (he == hipSuccess) {
         int bits = 1;
         while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
         c->sort_bits = bits;
         c->sort_temp = sort_temp_bytes(n, bits);
-        if (he == hipSuccess) he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
+        he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
     }
     if (he != hipSuccess) {
         free_particles(c);
@@ -524,10 +843,12 @@ int start_load_particles(start_ctx* c, int64_t n, const double* x, const double*
         c->ids.resize((size_t)n);
         for (int64_t i = 0; i < n; ++i) c->ids[(size_t)i] = (uint64_t)i;
     }
+    const int64_t CL = c->charge_len();
     HIPCHECK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_tri_count, 0, sizeof(unsigned long long) * std::max<int64_t>(c->hm.n_tris, 1),
+    HIPCHECK(c, hipMemsetAsync(c->d_tri_count, 0,
+                               sizeof(unsigned long long) * c->n_runs() * std::max<int64_t>(c->hm.n_tris, 1),
                                c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_node_charge, 0, sizeof(double) * 2 * c->hm.n_nodes, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_node_charge, 0, sizeof(double) * 2 * CL, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->cb = 0;
     c->latest = 0;
@@ -582,11 +903,26 @@ int start_load_particles(start_ctx* c, int64_t n, const double* x, const double*
         }
     }
     HIPCHECK(c, hipGetLastError());
-    if (c->prm.sort_interval > 0) {
-        int rc = sort_now(c);
-        if (rc != START_OK) return rc;
+    if (c->cfg.sort_interval > 0) {
+        int rc2 = sort_now(c);
+        if (rc2 != START_OK) return rc2;
     }
     return check_kernel_errors(c);
+}
+
+int start_load_particles(start_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                         const double* vx, const double* vy, const double* vz, const int32_t* species,
+                         const uint64_t* ids)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (n < 0 || n > (int64_t)0x7fffffff) return fail(c, START_ERR_ARG, "particle count out of range");
+    start_run_desc r{};
+    r.n_particles = n;
+    r.field_scale[0] = 1.0;
+    r.field_scale[1] = 0.0;
+    r.concentration = c->prm.gas.concentration;
+    r.run = c->prm.run;
+    return start_load_batch(c, 1, &r, x, y, z, vx, vy, vz, species, ids);
 }
 
 int start_set_gas(start_ctx* c, const start_gas_desc* g, const double* diameters)
@@ -607,6 +943,12 @@ int start_set_gas(start_ctx* c, const start_gas_desc* g, const double* diameters
     c->species_desc = sd;
     c->species = k;
     c->prm.gas = *g;
+    if (c->n_runs() == 1) {  // a single simulation takes the concentration with the gas
+        c->runs[0].conc = g->concentration;
+        int rc = upload_runs(c);
+        if (rc != START_OK) return rc;
+    }
+    drop_graphs(c);
     return START_OK;
 }
 
@@ -614,33 +956,74 @@ int start_set_time_step(start_ctx* c, double dt)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
     if (!(dt >= 0.0)) return fail(c, START_ERR_ARG, "time step must be >= 0");
+    if (dt != c->prm.dt) drop_graphs(c);
     c->prm.dt = dt;
     return START_OK;
 }
 
-int start_set_tet_field(start_ctx* c, const double* E)
+int start_set_tet_field_basis(start_ctx* c, int32_t which, const double* E)
 {
     if (!c || !E) return fail(c, START_ERR_ARG, "null argument");
-    HIPCHECK(c, hipMemcpyAsync(c->d_E, E, sizeof(double) * 3 * c->hm.n_tets, hipMemcpyHostToDevice, c->stream));
-    if (c->d_E32) launch_e32(c->stream, c->d_E, c->d_E32, c->hm.n_tets);
+    if (which != 0 && which != 1) return fail(c, START_ERR_ARG, "field basis must be 0 or 1");
+    const int64_t T = c->hm.n_tets;
+    if (which == 1 && !c->d_E1) {
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        HIPCHECK(c, dalloc(&c->d_E1, 3 * T));
+        if (c->mixed) HIPCHECK(c, dalloc(&c->d_E1_32, 4 * T));
+        drop_graphs(c);
+    }
+    double* dE = which ? c->d_E1 : c->d_E;
+    float* dE32 = which ? c->d_E1_32 : c->d_E32;
+    HIPCHECK(c, hipMemcpyAsync(dE, E, sizeof(double) * 3 * T, hipMemcpyHostToDevice, c->stream));
+    if (dE32) launch_e32(c->stream, dE, dE32, T);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return START_OK;
 }
 
-int start_set_node_potential(start_ctx* c, const double* phi)
+int start_set_tet_field(start_ctx* c, const double* E) { return start_set_tet_field_basis(c, 0, E); }
+
+int start_set_node_potential_basis(start_ctx* c, int32_t which, const double* phi)
 {
     if (!c || !phi) return fail(c, START_ERR_ARG, "null argument");
+    if (which != 0 && which != 1) return fail(c, START_ERR_ARG, "field basis must be 0 or 1");
+    const int64_t T = c->hm.n_tets;
+    if (which == 1 && !c->d_E1) {
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        HIPCHECK(c, dalloc(&c->d_E1, 3 * T));
+        if (c->mixed) HIPCHECK(c, dalloc(&c->d_E1_32, 4 * T));
+        drop_graphs(c);
+    }
     double* d_phi = nullptr;
     HIPCHECK(c, dalloc(&d_phi, c->hm.n_nodes));
     hipError_t e = hipMemcpyAsync(d_phi, phi, sizeof(double) * c->hm.n_nodes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        launch_field(c->stream, c->mesh_view(), d_phi, c->d_E, c->d_E32);
+        launch_field(c->stream, c->mesh_view(), d_phi, which ? c->d_E1 : c->d_E, which ? c->d_E1_32 : c->d_E32);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d_phi);
     if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("field from potential: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+int start_set_node_potential(start_ctx* c, const double* phi) { return start_set_node_potential_basis(c, 0, phi); }
+
+int start_set_launch(start_ctx* c, const start_launch_cfg* cfg)
+{
+    if (!c || !cfg) return fail(c, START_ERR_ARG, "null argument");
+    int rc = check_cfg(c, *cfg);
+    if (rc != START_OK) return rc;
+    c->cfg = *cfg;
+    c->prm.steps_per_launch = cfg->steps_per_launch;
+    c->prm.sort_interval = cfg->sort_interval;
+    return START_OK;
+}
+
+int start_get_launch(start_ctx* c, start_launch_cfg* cfg)
+{
+    if (!c || !cfg) return fail(c, START_ERR_ARG, "null argument");
+    *cfg = c->cfg;
     return START_OK;
 }
 
@@ -652,85 +1035,123 @@ int start_step(start_ctx* c, int32_t nsteps)
         c->step += (uint64_t)nsteps;
         return START_OK;
     }
-    DevMesh M = c->mesh_view();
-    StepParams S{};
-    S.dt = c->prm.dt;
-    S.conc = c->prm.gas.concentration;
-    for (int k = 0; k < 3; ++k) {
-        S.lo[k] = c->hm.bbox_lo[k];
-        S.hi[k] = c->hm.bbox_hi[k];
-    }
-    S.seed = c->prm.seed;
-    S.run = c->prm.run;
-    S.model = c->prm.gas.model;
-    S.em_push = c->prm.em_push;
-    S.charge = c->prm.charge_every_step;
-    S.nspecies = (int32_t)c->species.size();
-    S.n_active = c->d_nactive;
-    const int64_t Mn = c->hm.n_nodes;
-    const int32_t K = c->prm.steps_per_launch;
+    // Launches of steps_per_launch steps; before a launch, a sort if
+    // sort_interval steps have passed since the last one.  A chunk is the run
+    // of launches between two sorts.
     int32_t done = 0;
+    const int32_t SI = c->cfg.sort_interval;
+    const int32_t K = std::max(1, c->cfg.steps_per_launch);
     while (done < nsteps) {
-        if (c->prm.sort_interval > 0 && c->step - c->last_sort >= (uint64_t)c->prm.sort_interval) {
+        if (SI > 0 && c->step - c->last_sort >= (uint64_t)SI) {
             int rc = sort_now(c);
             if (rc != START_OK) return rc;
         }
-        int32_t kk = std::min(K, nsteps - done);
-        double* nc = c->prm.charge_every_step ? c->d_node_charge + (size_t)c->cb * Mn : nullptr;
-        S.zero_charge = c->prm.charge_every_step ? c->d_node_charge + (size_t)(1 - c->cb) * Mn : nullptr;
-        S.work_slot = c->wslot;
-        S.step = (uint32_t)c->step;
-        S.nsteps = kk;
-        TimedStep* ev = nullptr;
-        if (c->timing) {
-            if (c->ev_used == c->events.size()) {
-                TimedStep t{};
-                HIPCHECK(c, hipEventCreate(&t.a0));
-                HIPCHECK(c, hipEventCreate(&t.a1));
-                HIPCHECK(c, hipEventCreate(&t.b1));
-                c->events.push_back(t);
-            }
-            ev = &c->events[c->ev_used++];
+        int32_t chunk = nsteps - done;
+        if (SI > 0) {
+            const int64_t to_sort = (int64_t)SI - (int64_t)(c->step - c->last_sort);
+            chunk = (int32_t)std::min<int64_t>(chunk, (to_sort + K - 1) / K * K);
         }
-        if (ev) HIPCHECK(c, hipEventRecord(ev->a0, c->stream));
-        if (c->mixed)
-            launch_step<float>(c->stream, M, c->view<float>(), S, c->d_species, nc, c->d_wl, c->d_cnt,
-                               c->d_tri_count, ev ? ev->a1 : nullptr);
-        else
-            launch_step<double>(c->stream, M, c->view<double>(), S, c->d_species, nc, c->d_wl, c->d_cnt,
-                                c->d_tri_count, ev ? ev->a1 : nullptr);
-        if (ev) HIPCHECK(c, hipEventRecord(ev->b1, c->stream));
-        HIPCHECK(c, hipGetLastError());
-        c->step += (uint64_t)kk;
-        done += kk;
-        c->wslot ^= 1;
-        if (nc) {
-            c->latest = c->cb;
-            c->cb ^= 1;
-        }
+        int rc = (c->cfg.graph && !c->timing) ? run_chunk_graph(c, chunk) : enqueue_chunk(c, chunk);
+        if (rc != START_OK) return rc;
+        c->step += (uint64_t)chunk;
+        done += chunk;
     }
     return START_OK;
+}
+
+int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, double* ms_per_step)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (trial_steps < 1) return fail(c, START_ERR_ARG, "trial_steps must be >= 1");
+    if (c->n == 0) return fail(c, START_ERR_STATE, "no particles loaded");
+    const start_launch_cfg keep = c->cfg;
+    const bool timing = c->timing;
+    c->timing = false;
+    Snapshot snap;
+    int rc = snapshot_take(c, snap);
+    if (rc != START_OK) {
+        snapshot_free(snap);
+        c->timing = timing;
+        return rc;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    start_launch_cfg best_cfg = keep;
+    double best_ms = -1.0;
+    const int32_t Ks[] = {10, 25, 50, 100, 200};
+    const int32_t blocks[] = {128, 256};
+    for (int32_t K : Ks) {
+        if (K > trial_steps) continue;
+        for (int32_t bs : blocks)
+            for (int32_t pers = 0; pers <= 1 && rc == START_OK; ++pers)
+                for (int32_t sm = 1; sm <= 4 && rc == START_OK; sm *= 2) {
+                    start_launch_cfg g{K, bs, pers, K * sm, keep.graph, 0};
+                    if (g.sort_interval > trial_steps) continue;
+                    rc = snapshot_restore(c, snap);
+                    if (rc != START_OK) break;
+                    c->cfg = g;
+                    (void)hipEventRecord(e0, c->stream);
+                    rc = start_step(c, trial_steps);
+                    (void)hipEventRecord(e1, c->stream);
+                    if (rc != START_OK) break;
+                    if (hipEventSynchronize(e1) != hipSuccess) {
+                        rc = fail(c, START_ERR_HIP, "autotune timing");
+                        break;
+                    }
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    double per = ms / trial_steps;
+                    if (best_ms < 0.0 || per < best_ms) {
+                        best_ms = per;
+                        best_cfg = g;
+                    }
+                }
+    }
+    int rc2 = snapshot_restore(c, snap);
+    snapshot_free(snap);
+    drop_graphs(c);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->timing = timing;
+    c->cfg = rc == START_OK ? best_cfg : keep;
+    c->prm.steps_per_launch = c->cfg.steps_per_launch;
+    c->prm.sort_interval = c->cfg.sort_interval;
+    if (rc != START_OK) return rc;
+    if (rc2 != START_OK) return rc2;
+    if (best) *best = best_cfg;
+    if (ms_per_step) *ms_per_step = best_ms;
+    return check_kernel_errors(c);
 }
 
 int start_gather_charge(start_ctx* c)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
     if (c->n == 0) return START_OK;
-    const int64_t Mn = c->hm.n_nodes;
-    double* cur = c->d_node_charge + (size_t)c->cb * Mn;
-    launch_zero_charge(c->stream, cur, c->d_node_charge + (size_t)(1 - c->cb) * Mn, Mn);
-    if (c->mixed) launch_gather<float>(c->stream, c->mesh_view(), c->view<float>(), c->d_species, cur);
-    else launch_gather<double>(c->stream, c->mesh_view(), c->view<double>(), c->d_species, cur);
+    const int64_t CL = c->charge_len();
+    double* cur = c->d_node_charge + (size_t)c->cb * CL;
+    launch_zero_charge(c->stream, cur, c->d_node_charge + (size_t)(1 - c->cb) * CL, CL);
+    if (c->mixed) launch_gather<float>(c->stream, c->mesh_view(), c->view<float>(), c->d_species, c->d_runs, cur);
+    else launch_gather<double>(c->stream, c->mesh_view(), c->view<double>(), c->d_species, c->d_runs, cur);
     HIPCHECK(c, hipGetLastError());
     c->latest = c->cb;
     c->cb ^= 1;
     return START_OK;
 }
 
+int start_read_node_charge_runs(start_ctx* c, double* out)
+{
+    if (!c || !out) return fail(c, START_ERR_ARG, "null argument");
+    HIPCHECK(c, hipMemcpyAsync(out, c->d_node_charge + (size_t)c->latest * c->charge_len(),
+                               sizeof(double) * c->charge_len(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return check_kernel_errors(c);
+}
+
 int start_read_node_charge(start_ctx* c, double* out)
 {
     if (!c || !out) return fail(c, START_ERR_ARG, "null argument");
-    HIPCHECK(c, hipMemcpyAsync(out, c->d_node_charge + (size_t)c->latest * c->hm.n_nodes,
+    HIPCHECK(c, hipMemcpyAsync(out, c->d_node_charge + (size_t)c->latest * c->charge_len(),
                                sizeof(double) * c->hm.n_nodes, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return check_kernel_errors(c);
@@ -741,6 +1162,7 @@ int start_node_charge_density(start_ctx* c, int32_t mode, double* rho, int32_t* 
     if (!c || !rho) return fail(c, START_ERR_ARG, "null argument");
     if (mode != START_DENSITY_BARYCENTRIC && mode != START_DENSITY_TET_AVERAGE)
         return fail(c, START_ERR_ARG, "unknown density mode");
+    if (c->n_runs() != 1) return fail(c, START_ERR_STATE, "node charge density is defined for a single run");
     int rc = ensure_density_tables(c);
     if (rc != START_OK) return rc;
     const int64_t M = c->hm.n_nodes, T = c->hm.n_tets;
@@ -843,6 +1265,16 @@ int start_read_tri_counts(start_ctx* c, uint64_t* counts)
     if (c->hm.n_tris == 0) return START_OK;
     HIPCHECK(c, hipMemcpyAsync(counts, c->d_tri_count, sizeof(uint64_t) * c->hm.n_tris, hipMemcpyDeviceToHost,
                                c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return START_OK;
+}
+
+int start_read_tri_counts_runs(start_ctx* c, uint64_t* counts)
+{
+    if (!c || !counts) return fail(c, START_ERR_ARG, "null argument");
+    if (c->hm.n_tris == 0) return START_OK;
+    HIPCHECK(c, hipMemcpyAsync(counts, c->d_tri_count, sizeof(uint64_t) * c->n_runs() * c->hm.n_tris,
+                               hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return START_OK;
 }

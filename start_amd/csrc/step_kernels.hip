@@ -319,31 +319,31 @@ __device__ inline void scatter4(double* __restrict__ node_charge, const int4& nv
 // linear probing) and flushed with one FP64 atomic per distinct node.  With
 // particles sorted by tet a 256-particle block touches ~100-150 nodes instead
 // of issuing 1,024 global atomics.
-constexpr int kHashBits = 11;
-constexpr int kHashSlots = 1 << kHashBits;
-
+template <int Bits>
 __device__ inline void lds_charge_add(int32_t* hk, double* hv, double* __restrict__ node_charge, int32_t node,
                                       double val)
 {
-    uint32_t h = ((uint32_t)node * 2654435761u) >> (32 - kHashBits);
+    uint32_t h = ((uint32_t)node * 2654435761u) >> (32 - Bits);
     for (int probe = 0; probe < 32; ++probe) {
         int32_t prev = atomicCAS(&hk[h], -1, node);
         if (prev == -1 || prev == node) {
             atomicAdd(&hv[h], val);
             return;
         }
-        h = (h + 1) & (kHashSlots - 1);
+        h = (h + 1) & ((1u << Bits) - 1u);
     }
     unsafeAtomicAdd(&node_charge[node], val);  // crowded table: straight to memory
 }
 
+// base: offset of the run's node-charge array (0 for a single simulation)
+template <int Bits>
 __device__ inline void lds_scatter4(int32_t* hk, double* hv, double* __restrict__ node_charge, const int4& nv,
-                                    double q, const double lam[4])
+                                    int32_t base, double q, const double lam[4])
 {
-    lds_charge_add(hk, hv, node_charge, nv.x, q * lam[0]);
-    lds_charge_add(hk, hv, node_charge, nv.y, q * lam[1]);
-    lds_charge_add(hk, hv, node_charge, nv.z, q * lam[2]);
-    lds_charge_add(hk, hv, node_charge, nv.w, q * lam[3]);
+    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.x, q * lam[0]);
+    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.y, q * lam[1]);
+    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.z, q * lam[2]);
+    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.w, q * lam[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,6 +367,40 @@ __device__ inline void load_E(const DevMesh& M, int32_t t, float& ex, float& ey,
     ex = e.x;
     ey = e.y;
     ez = e.z;
+}
+__device__ inline void load_E1(const DevMesh& M, int32_t t, double& ex, double& ey, double& ez)
+{
+    const double* E = M.E1 + 3 * (int64_t)t;
+    ex = E[0];
+    ey = E[1];
+    ez = E[2];
+}
+__device__ inline void load_E1(const DevMesh& M, int32_t t, float& ex, float& ey, float& ez)
+{
+    float4 e = reinterpret_cast<const float4*>(M.E1_32)[t];
+    ex = e.x;
+    ey = e.y;
+    ez = e.z;
+}
+// field of the particle's run in tet t: s0 * E_0 + s1 * E_1 (s0 = 1 and no
+// second basis for a single simulation, where this is E_0 exactly)
+template <typename R, bool kBatch>
+__device__ inline void run_field(const DevMesh& M, const RunK* __restrict__ rk, int32_t t, R& ex, R& ey, R& ez)
+{
+    load_E(M, t, ex, ey, ez);
+    if (!kBatch) return;
+    const R s0 = (R)rk->s0;
+    ex = s0 * ex;
+    ey = s0 * ey;
+    ez = s0 * ez;
+    if (M.E1) {
+        R fx, fy, fz;
+        load_E1(M, t, fx, fy, fz);
+        const R s1 = (R)rk->s1;
+        ex = ex + s1 * fx;
+        ey = ey + s1 * fy;
+        ez = ez + s1 * fz;
+    }
 }
 __device__ inline void bary_at(const DevMesh& M, int32_t t, double x, double y, double z, double lam[4])
 {
@@ -453,29 +487,38 @@ __device__ inline void to_frame(const DevMesh& M, int32_t t, const double* p, fl
 // ---------------------------------------------------------------------------
 // charge sinks: LDS hash (streaming kernel) or direct atomics (rare paths)
 // ---------------------------------------------------------------------------
+template <int Bits>
 struct LdsSink {
     int32_t* hk;
     double* hv;
     double* nc;
-    __device__ void add4(const int4& nv, double q, const double lam[4]) { lds_scatter4(hk, hv, nc, nv, q, lam); }
+    __device__ void add4(const int4& nv, int32_t base, double q, const double lam[4])
+    {
+        lds_scatter4<Bits>(hk, hv, nc, nv, base, q, lam);
+    }
 };
 struct GlobalSink {
     double* nc;
-    __device__ void add4(const int4& nv, double q, const double lam[4]) { scatter4(nc, nv, q, lam); }
+    __device__ void add4(const int4& nv, int32_t base, double q, const double lam[4])
+    {
+        scatter4(nc + base, nv, q, lam);
+    }
 };
 // Per-particle accumulation across the steps of a launch: a particle that
 // stays in one tet adds q*lambda into four registers, and its sums go to the
-// inner sink only when it changes tet or the launch ends.
+// inner sink only when it changes tet or the launch ends.  base is the offset
+// of the particle's run in the node-charge buffer.
 template <typename Inner>
 struct RegSink {
     Inner inner;
     int32_t cur;
+    int32_t base;
     double a0, a1, a2, a3;
     __device__ void flush(const DevMesh& M)
     {
         if (cur >= 0) {
             double v[4] = {a0, a1, a2, a3};
-            inner.add4(M.tnode[cur], 1.0, v);
+            inner.add4(M.tnode[cur], base, 1.0, v);
         }
         cur = -1;
         a0 = a1 = a2 = a3 = 0.0;
@@ -506,24 +549,24 @@ __device__ __noinline__ bool collides_exact64(int model, double sigma_hs, double
 }
 
 template <typename R>
-__device__ inline bool collides(const StepParams& S, const SpeciesK& k, R vx, R vy, R vz, double u1)
+__device__ inline bool collides(const StepParams& S, const SpeciesK& k, double conc, R vx, R vy, R vz, double u1)
 {
     R vm = m_sqrt<R>(vx * vx + vy * vy + vz * vz);
     if (S.model == 1) {  // HS: no transcendental, evaluate directly
         CollideK<R> ck{(R)k.sigma_hs, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0, (R)0};
-        return (R)u1 < collision_probability<R>(1, ck, (R)S.conc, (R)S.dt, vm);
+        return (R)u1 < collision_probability<R>(1, ck, (R)conc, (R)S.dt, vm);
     }
     float vmf = (float)vm;
     CollideK<float> cf{(float)k.sigma_hs, (float)k.vhs_c, (float)k.two_kt, (float)k.m_r, (float)k.vhs_exp,
                        0.0f, 0.0f, 0.0f};
-    float p32 = collision_probability<float>(S.model, cf, (float)S.conc, (float)S.dt, vmf);
+    float p32 = collision_probability<float>(S.model, cf, (float)conc, (float)S.dt, vmf);
     if (!std::is_same<R, double>::value) return (float)u1 < p32;
     double p = (double)p32;
     if (!(p > 1e-30 && p < 1e30))  // outside the range the FP32 bound covers
-        return collides_exact64(S.model, k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, S.conc, S.dt, (double)vm, u1);
+        return collides_exact64(S.model, k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, conc, S.dt, (double)vm, u1);
     if (u1 < p * (1.0 - 1e-4)) return true;
     if (u1 >= p * (1.0 + 1e-4)) return false;
-    return collides_exact64(S.model, k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, S.conc, S.dt, (double)vm, u1);
+    return collides_exact64(S.model, k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, conc, S.dt, (double)vm, u1);
 }
 
 // post-collision velocity (rare: out of line to keep the streaming kernel
@@ -582,10 +625,10 @@ struct StepResult {
     int32_t hit_tri;
 };
 
-template <typename R, typename Sink>
+template <typename R, bool kBatch, typename Sink>
 __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParams& S, const SpeciesK& k,
-                                               uint32_t pidx, uint32_t step, int remaining, PState<R>& s,
-                                               Sink& sink)
+                                               const RunK* __restrict__ rk, uint32_t vid, uint32_t step,
+                                               int remaining, PState<R>& s, Sink& sink)
 {
     const double tau = tau_of<R>(M);
     // 2. |v| < 10 m/s and 3. outside the expanded bbox: frozen this step and
@@ -604,7 +647,7 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
     // 4. push with the owning tet's field (ParticleDynamicsProcessor.cpp:179-180)
     if (S.em_push && s.t >= 0) {
         R ex, ey, ez;
-        load_E(M, s.t, ex, ey, ez);
+        run_field<R, kBatch>(M, rk, s.t, ex, ey, ez);
         push<R>(ex, ey, ez, (R)k.charge, (R)k.mass, dt, s.vx, s.vy, s.vz);
     }
     // 6. move (Particle.cpp:157-165)
@@ -619,10 +662,11 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
     // 7. gas collision (ParticlePhysicsUpdater.cpp:20-24)
     if (S.model != 0) {
         double u1, u1b;
-        variates(S.seed, S.run, pidx, step, 0u, u1, u1b);
-        if (collides<R>(S, k, s.vx, s.vy, s.vz, u1)) {
+        const uint32_t key = rk->key;
+        variates(S.seed, key, vid, step, 0u, u1, u1b);
+        if (collides<R>(S, k, rk->conc, s.vx, s.vy, s.vz, u1)) {
             double u2, u3;
-            variates(S.seed, S.run, pidx, step, 1u, u2, u3);
+            variates(S.seed, key, vid, step, 1u, u2, u3);
             V3<R> nv = collide_velocity<R>(S.model, k.mass_cp, k.mass_ct, k.inv_alpha, u2, u3, s.vx, s.vy, s.vz);
             s.vx = nv.x;
             s.vy = nv.y;
@@ -713,38 +757,28 @@ __global__ void prologue_kernel(double* __restrict__ a, double* __restrict__ b, 
     }
 }
 
-// Streaming kernel: one thread per particle, S.nsteps steps with the state and
-// the owning tet's data in registers.  A particle whose relocation cannot be
-// certified stops and goes to the work list with the step it stopped at.
-template <typename R>
-__global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P, StepParams S,
-                                                 const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
-                                                 WorkItem* __restrict__ wl, unsigned long long* __restrict__ cnt,
-                                                 unsigned long long* __restrict__ tri_count)
+// One tile of the streaming kernel: BS consecutive particles, one per thread,
+// S.nsteps steps with the state and the owning tet's data in registers.  A
+// particle whose relocation cannot be certified stops and goes to the work
+// list with the step it stopped at.  Called with a block-uniform tile.
+template <typename R, int BS, int HashBits, bool kBatch>
+__device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R>& P, const StepParams& S,
+                                          const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
+                                          WorkItem* __restrict__ wl, unsigned long long* __restrict__ cnt,
+                                          unsigned long long* __restrict__ tri_count, int64_t tile, int64_t n_active,
+                                          uint32_t step0, unsigned int* blk_processed, int32_t* hk, double* hv)
 {
-    __shared__ unsigned int blk_processed;
-    __shared__ int32_t hk[kHashSlots];
-    __shared__ double hv[kHashSlots];
-    {
-        // clear the next launch's charge buffer and work-list counter
-        int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-        int64_t gs = (int64_t)gridDim.x * blockDim.x;
-        if (S.zero_charge)
-            for (int64_t j = g; j < M.n_nodes; j += gs) S.zero_charge[j] = 0.0;
-        if (g == 0) cnt[kCntWork + 2 * (1 - S.work_slot)] = 0ull;
-    }
-    const int64_t n_active = *S.n_active;
-    if ((int64_t)blockIdx.x * blockDim.x >= n_active) return;  // whole block past the live particles
-    if (threadIdx.x == 0) blk_processed = 0u;
+    constexpr int kSlots = 1 << HashBits;
+    if (threadIdx.x == 0) *blk_processed = 0u;
     if (S.charge) {
-        for (int h = threadIdx.x; h < kHashSlots; h += blockDim.x) {
+        for (int h = threadIdx.x; h < kSlots; h += BS) {
             hk[h] = -1;
             hv[h] = 0.0;
         }
-        __syncthreads();
     }
-    RegSink<LdsSink> sink{LdsSink{hk, hv, node_charge}, -1, 0.0, 0.0, 0.0, 0.0};
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    __syncthreads();
+    RegSink<LdsSink<HashBits>> sink{LdsSink<HashBits>{hk, hv, node_charge}, -1, 0, 0.0, 0.0, 0.0, 0.0};
+    int64_t i = tile * BS + threadIdx.x;
     int processed = 0;
     bool deferred = false;
     int defer_k = 0;
@@ -755,6 +789,9 @@ __global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P
         s.f = P.flags[i];
         if (!(s.f & kSettled)) {
             const SpeciesK& k = sp[(s.f >> kSpeciesShift) & 0xff];
+            // one simulation: run 0 for every lane (scalar loads); a batch: the lane's run
+            const RunK* rk = kBatch ? S.runs + ((uint32_t)s.f >> kRunShift) : S.runs;
+            sink.base = kBatch ? (int32_t)rk->node_base : 0;
             s.x = P.x[i];
             s.y = P.y[i];
             s.z = P.z[i];
@@ -763,19 +800,19 @@ __global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P
             s.vz = P.vz[i];
             s.t = P.tet[i];
             const int32_t t_in = s.t, f_in = s.f;
-            const uint32_t pidx = S.model != 0 ? P.pidx[i] : 0u;
+            const uint32_t vid = S.model != 0 ? P.pidx[i] - rk->first : 0u;
             bool moved = false;
             for (int ks = 0; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                StepResult r = step_one<R>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink);
+                StepResult r = step_one<R, kBatch>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (r.outcome == kFrozenNow) break;
                 ++processed;
                 moved = true;
                 if (r.outcome == kSettledNow) {
                     s.f = (s.f | kSettled) & ~kStrict;
                     P.hit[i] = r.hit_tri;
-                    atomicAdd(&tri_count[r.hit_tri], 1ull);
+                    atomicAdd(&tri_count[rk->tri_base + r.hit_tri], 1ull);
                     atomicAdd(&cnt[kCntSettled], 1ull);
                     break;
                 }
@@ -821,20 +858,68 @@ __global__ void __launch_bounds__(256, 4) step_main(DevMesh M, DevParticles<R> P
             wl[base + (unsigned)__popcll(dm & ((1ull << lane) - 1ull))] = it;
         }
     }
-    // processed count: wavefront sum, LDS per block, one atomic per block spread over slots
+    // processed count: wavefront sum, LDS per block, one atomic per tile spread over slots
     __syncthreads();
     int wsum = processed;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off);
-    if ((threadIdx.x & 63) == 0 && wsum) atomicAdd(&blk_processed, (unsigned int)wsum);
+    if ((threadIdx.x & 63) == 0 && wsum) atomicAdd(blk_processed, (unsigned int)wsum);
     __syncthreads();
-    if (threadIdx.x == 0 && blk_processed)
-        atomicAdd(&cnt[kCntProcSlot0 + (blockIdx.x & (kProcSlots - 1))], (unsigned long long)blk_processed);
+    if (threadIdx.x == 0 && *blk_processed)
+        atomicAdd(&cnt[kCntProcSlot0 + (tile & (kProcSlots - 1))], (unsigned long long)*blk_processed);
     if (S.charge) {
-        for (int h = threadIdx.x; h < kHashSlots; h += blockDim.x) {
+        for (int h = threadIdx.x; h < kSlots; h += BS) {
             int32_t node = hk[h];
             if (node >= 0) unsafeAtomicAdd(&node_charge[node], hv[h]);
         }
+    }
+    __syncthreads();  // the LDS table and counter are reused by the next tile
+}
+
+// Streaming kernel.  With persistent == 0 workgroup b takes tile b; otherwise
+// a grid sized to the chip pulls tiles from a queue counter until the live
+// prefix is covered (one launch for every run of a batch, whatever their
+// sizes).  The LDS hash has 8 slots per thread (12 B each: 24 KB at BS = 256).
+template <typename R, int BS, bool kBatch, bool kPersistent>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) step_main(DevMesh M, DevParticles<R> P, StepParams S,
+                                                       const SpeciesK* __restrict__ sp,
+                                                       double* __restrict__ node_charge, WorkItem* __restrict__ wl,
+                                                       unsigned long long* __restrict__ cnt,
+                                                       unsigned long long* __restrict__ tri_count)
+{
+    constexpr int kBits = BS == 256 ? 11 : 10;
+    static_assert((1 << kBits) == 8 * BS, "8 hash slots per thread");
+    __shared__ unsigned int blk_processed;
+    __shared__ int64_t s_tile;
+    __shared__ int32_t hk[1 << kBits];
+    __shared__ double hv[1 << kBits];
+    const uint32_t step0 = (uint32_t)cnt[kCntStep + S.work_slot];
+    {
+        // clear the next launch's charge buffers and counters, and give it its first step
+        int64_t g = blockIdx.x * (int64_t)BS + threadIdx.x;
+        int64_t gs = (int64_t)gridDim.x * BS;
+        if (S.zero_charge)
+            for (int64_t j = g; j < S.charge_len; j += gs) S.zero_charge[j] = 0.0;
+        if (g == 0) {
+            cnt[kCntWork + 2 * (1 - S.work_slot)] = 0ull;
+            cnt[kCntStep + (1 - S.work_slot)] = (unsigned long long)step0 + (unsigned long long)S.nsteps;
+            cnt[kCntTile + (1 - S.work_slot)] = 0ull;
+        }
+    }
+    const int64_t n_active = *S.n_active;
+    if (!kPersistent) {
+        if ((int64_t)blockIdx.x * BS < n_active)  // else the whole block is past the live particles
+            step_tile<R, BS, kBits, kBatch>(M, P, S, sp, node_charge, wl, cnt, tri_count, blockIdx.x, n_active, step0,
+                                            &blk_processed, hk, hv);
+        return;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = (int64_t)atomicAdd(&cnt[kCntTile + S.work_slot], 1ull);
+        __syncthreads();
+        const int64_t tile = s_tile;
+        if (tile * BS >= n_active) break;  // queue drained: every workgroup exits here
+        step_tile<R, BS, kBits, kBatch>(M, P, S, sp, node_charge, wl, cnt, tri_count, tile, n_active, step0,
+                                        &blk_processed, hk, hv);
     }
 }
 
@@ -883,9 +968,10 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
                                                    unsigned long long* __restrict__ tri_count)
 {
     const uint64_t nw = cnt[kCntWork + 2 * S.work_slot];
+    const uint32_t step0 = (uint32_t)cnt[kCntStep + S.work_slot];
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nw;
          w += (uint64_t)gridDim.x * blockDim.x) {
-        RegSink<GlobalSink> sink{GlobalSink{node_charge}, -1, 0.0, 0.0, 0.0, 0.0};
+        RegSink<GlobalSink> sink{GlobalSink{node_charge}, -1, 0, 0.0, 0.0, 0.0, 0.0};
         WorkItem it = wl[w];
         int64_t i = it.i;
         PState<R> s;
@@ -898,17 +984,19 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
         s.vy = P.vy[i];
         s.vz = P.vz[i];
         const SpeciesK& k = sp[(s.f >> kSpeciesShift) & 0xff];
+        const RunK* rk = S.runs + ((uint32_t)s.f >> kRunShift);
+        sink.base = (int32_t)rk->node_base;
         int exh = 0, err = 0, nexact = 0;
         int32_t hit_tri = -1;
         double x0[3] = {it.x0, it.y0, it.z0};
         bool settled = finish_exact(M, S, k, x0, s, sink, hit_tri, exh, err);
         if (!settled) {
-            const uint32_t pidx = S.model != 0 ? P.pidx[i] : 0u;
+            const uint32_t vid = S.model != 0 ? P.pidx[i] - rk->first : 0u;
             unsigned long long extra = 0;
             for (int ks = it.k + 1; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                StepResult rr = step_one<R>(M, S, k, pidx, S.step + (uint32_t)ks, S.nsteps - ks, s, sink);
+                StepResult rr = step_one<R, true>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (rr.outcome == kFrozenNow) break;
                 ++extra;
                 if (rr.outcome == kDeferNow) {
@@ -928,7 +1016,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
         if (settled) {
             s.f = (s.f | kSettled) & ~kStrict;
             P.hit[i] = hit_tri;
-            atomicAdd(&tri_count[hit_tri], 1ull);
+            atomicAdd(&tri_count[rk->tri_base + hit_tri], 1ull);
             atomicAdd(&cnt[kCntSettled], 1ull);
         }
         sink.flush(M);
@@ -980,7 +1068,7 @@ __global__ void locate_kernel(DevMesh M, const double* __restrict__ x, const dou
 
 template <typename R>
 __global__ void gather_kernel(DevMesh M, DevParticles<R> P, const SpeciesK* __restrict__ sp,
-                              double* __restrict__ node_charge)
+                              const RunK* __restrict__ runs, double* __restrict__ node_charge)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= P.n) return;
@@ -991,7 +1079,7 @@ __global__ void gather_kernel(DevMesh M, DevParticles<R> P, const SpeciesK* __re
     if (q == 0.0) return;
     double lam[4];
     bary_at(M, t, P.x[i], P.y[i], P.z[i], lam);
-    scatter4(node_charge, M.tnode[t], q, lam);
+    scatter4(node_charge + runs[(uint32_t)f >> kRunShift].node_base, M.tnode[t], q, lam);
 }
 
 // Occupancy and summed charge of each tet (one owner per unsettled particle).
@@ -1107,9 +1195,25 @@ void launch_zero_charge(hipStream_t s, double* a, double* b, int64_t n_nodes)
 template <typename R>
 void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const StepParams& S,
                  const SpeciesK* sp, double* node_charge, WorkItem* wl, unsigned long long* cnt,
-                 unsigned long long* tri_count, hipEvent_t ev_mid)
+                 unsigned long long* tri_count, const StepLaunch& L, hipEvent_t ev_mid)
 {
-    if (P.n > 0) step_main<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
+    if (P.n > 0) {
+        const unsigned grid = L.persistent ? (unsigned)L.grid : blocks_for(P.n, L.block == 128 ? 128 : 256);
+        const int v = (L.block == 128 ? 4 : 0) + (L.batch ? 2 : 0) + (L.persistent ? 1 : 0);
+        switch (v) {
+#define SG_STEP_MAIN(BS_, B_, P_) \
+    step_main<R, BS_, B_, P_><<<grid, BS_, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count)
+        case 0: SG_STEP_MAIN(256, false, false); break;
+        case 1: SG_STEP_MAIN(256, false, true); break;
+        case 2: SG_STEP_MAIN(256, true, false); break;
+        case 3: SG_STEP_MAIN(256, true, true); break;
+        case 4: SG_STEP_MAIN(128, false, false); break;
+        case 5: SG_STEP_MAIN(128, false, true); break;
+        case 6: SG_STEP_MAIN(128, true, false); break;
+        default: SG_STEP_MAIN(128, true, true); break;
+#undef SG_STEP_MAIN
+        }
+    }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
     unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 64u));
     step_resolve<R><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
@@ -1122,9 +1226,10 @@ void launch_locate(hipStream_t s, const DevMesh& M, const double* x, const doubl
 }
 
 template <typename R>
-void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp, double* node_charge)
+void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp, const RunK* runs,
+                   double* node_charge)
 {
-    if (P.n > 0) gather_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, sp, node_charge);
+    if (P.n > 0) gather_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, sp, runs, node_charge);
 }
 
 template <typename R>
@@ -1171,13 +1276,13 @@ void launch_abs_positions(hipStream_t s, const DevMesh& M, const float* lx, cons
 
 template void launch_step<double>(hipStream_t, const DevMesh&, const DevParticles<double>&, const StepParams&,
                                   const SpeciesK*, double*, WorkItem*, unsigned long long*, unsigned long long*,
-                                  hipEvent_t);
+                                  const StepLaunch&, hipEvent_t);
 template void launch_step<float>(hipStream_t, const DevMesh&, const DevParticles<float>&, const StepParams&,
                                  const SpeciesK*, double*, WorkItem*, unsigned long long*, unsigned long long*,
-                                 hipEvent_t);
+                                 const StepLaunch&, hipEvent_t);
 template void launch_gather<double>(hipStream_t, const DevMesh&, const DevParticles<double>&, const SpeciesK*,
-                                    double*);
+                                    const RunK*, double*);
 template void launch_gather<float>(hipStream_t, const DevMesh&, const DevParticles<float>&, const SpeciesK*,
-                                   double*);
+                                   const RunK*, double*);
 
 }  // namespace sg

@@ -39,7 +39,8 @@ class ParticleInCellStep:
     def __init__(self, mesh: TetMesh, species: list[Species], gas: Gas, dt: float, seed: int,
                  run: int = 0, em_push: bool = True, precision: str = "fp64",
                  charge_every_step: bool = True, bbox_expansion: float = BBOX_EXPANSION,
-                 steps_per_launch: int = 1, sort_interval: int = 20):
+                 steps_per_launch: int = 1, sort_interval: int = 20, block: int = 256,
+                 persistent: bool = False, graph: bool = False):
         L = _lib.lib()
         self.mesh = mesh
         self._nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
@@ -57,6 +58,10 @@ class ParticleInCellStep:
         self.ctx = C.c_void_p()
         _lib.check(L.start_create(C.byref(self.ctx), C.byref(md), C.byref(prm)), None)
         self.n = 0
+        self.n_runs = 1
+        if block != 256 or persistent or graph:
+            self.set_launch(steps_per_launch=steps_per_launch, block=block, persistent=persistent,
+                            sort_interval=sort_interval, graph=graph)
 
     def close(self):
         if self.ctx:
@@ -86,6 +91,66 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_load_particles(self.ctx, len(pos), *[_dp(c) for c in cols],
                                                     _ip(sp), idp))
         self.n = len(pos)
+        self.n_runs = 1
+
+    def load_batch(self, runs: list[dict], pos: np.ndarray, vel: np.ndarray, species: np.ndarray, ids=None):
+        """Independent runs on the shared mesh, their rows consecutive in run
+        order.  Each run: {"n": rows, "scale": (s0, s1), "concentration": n,
+        "run": variate key}; its field is s0 * basis 0 + s1 * basis 1."""
+        pos = np.asarray(pos, dtype=np.float64)
+        vel = np.asarray(vel, dtype=np.float64)
+        rd = (_lib.RunDesc * len(runs))()
+        for k, r in enumerate(runs):
+            s = r.get("scale", (1.0, 0.0))
+            rd[k] = _lib.RunDesc(int(r["n"]), (C.c_double * 2)(float(s[0]), float(s[1])),
+                                 float(r["concentration"]), int(r.get("run", k)), 0)
+        assert sum(int(r["n"]) for r in runs) == len(pos)
+        cols = [np.ascontiguousarray(a) for a in (pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0],
+                                                  vel[:, 1], vel[:, 2])]
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        idp = None
+        if ids is not None:
+            self._ids = np.ascontiguousarray(ids, dtype=np.uint64)
+            idp = _up(self._ids)
+        self._check(_lib.lib().start_load_batch(self.ctx, len(runs), rd, *[_dp(c) for c in cols], _ip(sp), idp))
+        self.n = len(pos)
+        self.n_runs = len(runs)
+
+    def set_field_basis(self, which: int, phi: np.ndarray | None = None, E: np.ndarray | None = None):
+        if phi is not None:
+            phi = np.ascontiguousarray(phi, dtype=np.float64)
+            self._check(_lib.lib().start_set_node_potential_basis(self.ctx, int(which), _dp(phi)))
+        else:
+            E = np.ascontiguousarray(E, dtype=np.float64).reshape(-1)
+            self._check(_lib.lib().start_set_tet_field_basis(self.ctx, int(which), _dp(E)))
+
+    def node_charge_runs(self) -> np.ndarray:
+        out = np.empty((self.n_runs, self.mesh.n_nodes), dtype=np.float64)
+        self._check(_lib.lib().start_read_node_charge_runs(self.ctx, _dp(out)))
+        return out
+
+    def tri_counts_runs(self) -> np.ndarray:
+        out = np.zeros((self.n_runs, max(self.mesh.n_tris, 1)), dtype=np.uint64)
+        self._check(_lib.lib().start_read_tri_counts_runs(self.ctx, _up(out)))
+        return out[:, : self.mesh.n_tris]
+
+    def set_launch(self, steps_per_launch=1, block=256, persistent=False, sort_interval=20, graph=False):
+        cfg = _lib.LaunchCfg(int(steps_per_launch), int(block), int(persistent), int(sort_interval), int(graph), 0)
+        self._check(_lib.lib().start_set_launch(self.ctx, C.byref(cfg)))
+
+    def get_launch(self) -> dict:
+        cfg = _lib.LaunchCfg()
+        self._check(_lib.lib().start_get_launch(self.ctx, C.byref(cfg)))
+        return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_ if k != "pad"}
+
+    def autotune(self, trial_steps: int = 200):
+        """Project autotuner: times the launch configurations from the current
+        state (restored afterwards) and keeps the fastest; returns it and its
+        ms per step."""
+        cfg = _lib.LaunchCfg()
+        ms = C.c_double(0)
+        self._check(_lib.lib().start_autotune(self.ctx, int(trial_steps), C.byref(cfg), C.byref(ms)))
+        return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_ if k != "pad"}, ms.value
 
     def set_tet_field(self, E: np.ndarray):
         E = np.ascontiguousarray(E, dtype=np.float64).reshape(-1)

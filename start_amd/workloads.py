@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import meshgen
-from .species import EV_J, MASS, Gas, Species, gas, species
+from .species import EV_J, MASS, Gas, Species, concentration, gas, species
 
 
 @dataclass
@@ -79,3 +79,45 @@ def large_box(n_particles: int = 1_000_000, steps: int = 2000, cells: int = 50,
     phi = meshgen.linear_potential(mesh, 0.0, 500.0)
     return Workload("large_box", mesh, [ar], g, 1e-9, steps, seed, pos, vel,
                     np.zeros(n_particles, dtype=np.int32), phi)
+
+
+@dataclass
+class Sweep:
+    """Independent runs on one mesh: particle counts, electrode voltages and
+    pressures per run, rows of the runs consecutive in run order."""
+    name: str
+    mesh: meshgen.TetMesh
+    species: list
+    gas: Gas
+    dt: float
+    steps: int
+    seed: int
+    runs: list          # {"n", "scale": (volts, 0), "concentration", "run", "pressure"}
+    pos: np.ndarray
+    vel: np.ndarray
+    sp_idx: np.ndarray
+    phi_unit: np.ndarray  # potential with the electrode at 1 V (field basis 0)
+
+
+def sweep_box(n_runs: int = 64, steps: int = 500, cells: int = 50, seed: int = 1004,
+              n_min: float = 1e4, n_max: float = 2e6, count_scale: float = 1.0) -> Sweep:
+    """Voltage/pressure sweep on the large box mesh: per-run particle counts
+    log-uniform in [n_min, n_max] (expected total 2.4e7 for 64 runs), electrode
+    voltage uniform in [0, 1000] V (0 V at z = 0), Ar pressure uniform in
+    [0.1, 5] Pa at 300 K with VSS collisions (alpha 1.4f, omega 0.81f); Ar+
+    (+1e) 1 eV Maxwellian with uniform positions; dt 1e-9 s."""
+    mesh = meshgen.box_kuhn(cells, cells, cells, 0.5, 0.5, 0.5)
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_runs))
+    counts = np.maximum((counts * count_scale).astype(np.int64), 1)
+    volts = rng.uniform(0.0, 1000.0, n_runs)
+    press = rng.uniform(0.1, 5.0, n_runs)
+    ar = species("Ar", "Ar", charge_units=1.0, diameter=4.17e-10)
+    g = gas("Ar", 1.0, 300.0, "VSS")
+    n = int(counts.sum())
+    pos, vel = maxwellian_box(mesh, n, rng, MASS["Ar"], 1.0)
+    runs = [{"n": int(counts[r]), "scale": (float(volts[r]), 0.0),
+             "concentration": concentration(float(press[r]), 300.0), "run": r, "pressure": float(press[r])}
+            for r in range(n_runs)]
+    return Sweep("sweep_box", mesh, [ar], g, 1e-9, steps, seed, runs, pos, vel,
+                 np.zeros(n, dtype=np.int32), meshgen.linear_potential(mesh, 0.0, 1.0))

@@ -1,0 +1,163 @@
+"""Batched sweeps: independent runs of uneven size on one mesh in the same
+launches, each with its own field (a combination of two field bases), gas
+concentration and variate key.  Every run is checked against the oracle run
+on its own, to the FP64 bar; the launch shapes (workgroup size, persistent
+tile queue, captured hipGraphs) must not change a single bit."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from start_amd import meshgen, workloads
+from start_amd.engine import ParticleInCellStep
+from start_amd.species import E_CHARGE
+
+from ._parity import REL_Q, REL_XV, log
+
+pytestmark = pytest.mark.gpu
+
+
+class BatchPair:
+    """A batch on the GPU and one oracle simulation per run."""
+
+    def __init__(self, sizes, scales, concs, keys, K=1, sort_interval=10, block=256, persistent=False,
+                 graph=False, precision="fp64", model="VSS", seed=1004):
+        w = workloads.large_box(n_particles=sum(sizes), cells=10, model=model, seed=seed)
+        w.dt = 2e-8
+        self.w, self.sizes, self.K = w, sizes, K
+        mesh = w.mesh
+        self.om = O.Mesh(mesh.nodes, mesh.tets, mesh.tris)
+        E0 = self.om.tet_field(meshgen.linear_potential(mesh, 0.0, 1.0))        # top electrode at 1 V
+        E1 = self.om.tet_field(meshgen.linear_potential(mesh, 0.0, 1.0, axis=0))  # side electrode at 1 V
+        g = w.gas
+        ks = [O.species_constants(s.mass, s.charge, s.diameter, g.mass, g.omega, g.alpha, O.MODEL[g.model])
+              for s in w.species]
+        self.sims, self.states, self.qo = [], [], []
+        off = 0
+        runs = []
+        for r, n in enumerate(sizes):
+            s0, s1 = scales[r]
+            E = s0 * E0 + s1 * E1
+            pos, vel = w.pos[off:off + n], w.vel[off:off + n]
+            tet0 = self.om.locate_many(pos[:, 0], pos[:, 1], pos[:, 2])
+            self.states.append(O.State.new(pos, vel, w.sp_idx[off:off + n], np.arange(n, dtype=np.uint64), tet0))
+            self.sims.append(O.Sim(self.om, ks, O.MODEL[g.model], concs[r], w.dt, w.seed, run=keys[r], E=E))
+            self.qo.append(np.zeros(mesh.n_nodes))
+            runs.append({"n": n, "scale": (s0, s1), "concentration": concs[r], "run": keys[r]})
+            off += n
+        self.eng = ParticleInCellStep(mesh, w.species, g, w.dt, w.seed, precision=precision,
+                                      steps_per_launch=K, sort_interval=sort_interval, block=block,
+                                      persistent=persistent, graph=graph)
+        self.eng.set_field_basis(0, E=E0)
+        self.eng.set_field_basis(1, E=E1)
+        self.eng.load_batch(runs, w.pos, w.vel, w.sp_idx)
+        self.step = 0
+
+    def advance(self, k):
+        for j in range(k):
+            for r in range(len(self.sizes)):
+                if j % self.K == 0:
+                    self.qo[r][:] = 0.0
+                self.sims[r].step(self.states[r], self.step, self.qo[r], nthreads=8)
+            self.step += 1
+        self.eng.process(k)
+
+    def compare(self, where):
+        g = self.eng.particles()
+        qg = self.eng.node_charge_runs()
+        tg = self.eng.tri_counts_runs()
+        off = 0
+        settled = 0
+        for r, n in enumerate(self.sizes):
+            st = self.states[r]
+            sl = slice(off, off + n)
+            for key, ref in (("tet", st.tet), ("status", st.status), ("hit", st.hit)):
+                bad = np.nonzero(g[key][sl] != ref)[0]
+                assert len(bad) == 0, f"{where}: run {r} {key} differs at {bad[:5]}"
+            xo = np.stack([st.x, st.y, st.z], 1)
+            xg = np.stack([g["x"][sl], g["y"][sl], g["z"][sl]], 1)
+            vo = np.stack([st.vx, st.vy, st.vz], 1)
+            vg = np.stack([g["vx"][sl], g["vy"][sl], g["vz"][sl]], 1)
+            ex = (np.abs(xo - xg).max(1) / np.linalg.norm(xo, axis=1)).max()
+            ev = (np.abs(vo - vg).max(1) / np.maximum(np.linalg.norm(vo, axis=1), 1e-300)).max()
+            assert ex <= REL_XV and ev <= REL_XV, f"{where}: run {r} x {ex:.2e} v {ev:.2e}"
+            assert np.array_equal(tg[r], self.sims[r].tri_count[: self.w.mesh.n_tris]), f"{where}: run {r} counts"
+            eq = (np.abs(qg[r] - self.qo[r]) / np.maximum(np.abs(self.qo[r]), E_CHARGE)).max()
+            assert eq <= REL_Q, f"{where}: run {r} charge {eq:.2e}"
+            settled += int((st.status == 1).sum())
+            off += n
+        log(f"{where}: {len(self.sizes)} runs ok, settled {settled}")
+        return settled
+
+    def close(self):
+        self.eng.close()
+
+
+SIZES = [3000, 11000, 500, 7000]
+SCALES = [(500.0, 0.0), (0.0, 0.0), (1000.0, -300.0), (125.5, 40.0)]
+CONCS = [2.4e22, 2.4e20, 1.2e23, 6.0e21]
+KEYS = [0, 1, 7, 1004]
+
+
+@pytest.mark.parametrize("K,block,persistent,graph", [(1, 256, False, False), (5, 128, True, True),
+                                                      (10, 256, True, False), (4, 128, False, True)])
+def test_batch_runs_match_independent_oracle_runs(K, block, persistent, graph):
+    bp = BatchPair(SIZES, SCALES, CONCS, KEYS, K=K, sort_interval=2 * K, block=block, persistent=persistent,
+                   graph=graph)
+    settled = 0
+    for _ in range(3):
+        bp.advance(20)
+        settled = bp.compare(f"batch K={K} block={block} persistent={persistent} graph={graph} step {bp.step}")
+    assert settled > 20
+    bp.close()
+
+
+def test_graph_replays_match_eager_and_oracle():
+    """A captured chunk replayed many times, with host reads between replays."""
+    w = workloads.small_box(n_particles=3000, charge_units=2.0)
+    from ._parity import Pair
+    pr = Pair(w, steps_per_launch=5, sort_interval=10)
+    pr.eng.set_launch(steps_per_launch=5, block=256, persistent=False, sort_interval=10, graph=True)
+    for k in range(6):
+        pr.advance(10)  # one chunk: two launches from the same graph each time
+        pr.compare(f"graph small_box step {pr.step}")
+    pr.close()
+
+
+def test_autotune_restores_state_and_keeps_parity():
+    bp = BatchPair([4000, 2500], [(500.0, 0.0), (200.0, 10.0)], [2.4e22, 6.0e21], [3, 4], K=1, sort_interval=10)
+    bp.advance(10)
+    cfg, ms = bp.eng.autotune(trial_steps=40)
+    log(f"autotune picked {cfg} at {ms:.4f} ms/step")
+    assert cfg["block"] in (128, 256) and cfg["steps_per_launch"] in (10, 25) and ms > 0
+    assert bp.eng.get_launch() == cfg
+    bp.K = cfg["steps_per_launch"]
+    bp.compare("after autotune, step 10")  # state as before the trials
+    bp.advance(50)
+    bp.compare("after autotune, step 60")
+    bp.close()
+
+
+def test_mixed_batch_against_fp64_oracle():
+    """The sweep's precision (FP32 push/collision/state, FP64 containment and
+    charge) per run against the FP64 oracle: owner/status agreement > 99.5%,
+    x within 1e-5, node charge L1 within 1e-4 of each run's total."""
+    bp = BatchPair(SIZES, SCALES, CONCS, KEYS, K=10, sort_interval=20, precision="mixed", persistent=True,
+                   graph=True)
+    bp.advance(100)
+    g = bp.eng.particles()
+    qg = bp.eng.node_charge_runs()
+    off = 0
+    for r, n in enumerate(SIZES):
+        st = bp.states[r]
+        sl = slice(off, off + n)
+        same = (g["tet"][sl] == st.tet) & (g["status"][sl] == st.status)
+        xo = np.stack([st.x, st.y, st.z], 1)[same]
+        xg = np.stack([g["x"][sl], g["y"][sl], g["z"][sl]], 1)[same]
+        ex = (np.abs(xo - xg).max(1) / np.linalg.norm(xo, axis=1)).max()
+        eq = np.abs(qg[r] - bp.qo[r]).sum() / max(np.abs(bp.qo[r]).sum(), E_CHARGE)
+        log(f"mixed batch run {r}: differ {1 - same.mean():.2e} x {ex:.2e} charge L1 {eq:.2e}")
+        assert 1 - same.mean() < 5e-3 and ex < 1e-5 and eq < 1e-4
+        off += n
+    bp.close()

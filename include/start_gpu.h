@@ -170,6 +170,26 @@ int start_get_launch(start_ctx* ctx, start_launch_cfg* cfg);
  * restores the state, and sets the fastest.  best and ms_per_step may be NULL. */
 int start_autotune(start_ctx* ctx, int32_t trial_steps, start_launch_cfg* best, double* ms_per_step);
 
+/* ---- adjoint of the collisionless push and the surface-hit map ----
+ * (no counterpart in the reference; gives d(deposition moments)/d(electrode
+ * potentials) for parameter optimisation).  The field of the run is
+ * s0 * E_0 + s1 * E_1 with (s0, s1) the run's field_scale, so the electrode
+ * potentials are the scales of two unit-potential field bases.
+ *
+ * start_record_settle(ctx, 1) makes the following steps record, per particle,
+ * the step in which it settled and the start of that step's segment.
+ * start_read_hits gives the hit point x_h = x_a + s (x_b - x_a) of each settled
+ * particle (x_a, x_b the ends of the segment, s where it meets the plane of the
+ * hit triangle), the triangle and the step; -1 for the others.
+ * start_adjoint_push takes lambda = dL/dx_h per particle (3n, load order) and
+ * returns dL/d(s0, s1) for steps first_step.. of a collisionless run, by a
+ * reverse sweep that recomputes the particles' owners backwards in time. */
+int start_record_settle(start_ctx* ctx, int32_t on);
+int start_read_hits(start_ctx* ctx, double* hit_xyz /* 3n */, int32_t* hit_tri /* n */,
+                    int32_t* settle_step /* n */);
+int start_adjoint_push(start_ctx* ctx, const double* lambda /* 3n */, int32_t first_step,
+                       double* grad /* 2 */);
+
 /* Replaces the background gas (model, target, concentration) and, when
  * diameters is not NULL, the species' collision diameters (n_species values,
  * r_p + r_t depends on the target); recomputes the collision constants.

@@ -26,7 +26,7 @@ EXPORTS = [
     "start_node_charge_density", "start_set_gas", "start_set_time_step",
     "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
-    "start_autotune",
+    "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
 ]
 DENSITY = {"barycentric": 0, "tet_average": 1}
 
@@ -116,6 +116,9 @@ def lib() -> C.CDLL:
     L.start_set_launch.argtypes = [vp, P(LaunchCfg)]
     L.start_get_launch.argtypes = [vp, P(LaunchCfg)]
     L.start_autotune.argtypes = [vp, i32, P(LaunchCfg), P(d)]
+    L.start_record_settle.argtypes = [vp, i32]
+    L.start_read_hits.argtypes = [vp, P(d), P(i32), P(i32)]
+    L.start_adjoint_push.argtypes = [vp, P(d), i32, P(d)]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is C.c_int:  # default

@@ -93,6 +93,15 @@ struct DevParticles {
     int64_t n;
 };
 
+// Where and when a particle settled, indexed by load index (recorded on
+// request, for the adjoint of the push and hit map): the absolute position at
+// the start of the step whose segment met the surface, and that step.
+struct SettleRec {
+    double x0, y0, z0;
+    int32_t step;  // -1: not settled
+    int32_t pad;
+};
+
 // Shape of a step launch: workgroup size and tile scheduling.
 struct StepLaunch {
     int block;       // 128 | 256 threads
@@ -115,6 +124,7 @@ struct StepParams {
     int64_t charge_len;       // n_runs * n_nodes
     int32_t work_slot;        // slot of this launch's work-list, step and tile counters
     int32_t nsteps;           // steps fused in this launch (charge accumulates over them)
+    SettleRec* settle;        // null, or one record per load index
 };
 
 }  // namespace sg

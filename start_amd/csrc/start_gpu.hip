@@ -30,6 +30,13 @@ void launch_node_density(hipStream_t s, const DevParticles<R>& P, const SpeciesK
                          const double* node_q, const double* node_vol, const int32_t* adj_start,
                          const int32_t* adj_list, const double* tet_vol, double* tet_q, int32_t* tet_n,
                          int64_t n_nodes, double* rho, int32_t* written);
+template <typename R>
+void launch_hit_points(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SettleRec* rec, double* xh,
+                       int32_t* tri, int32_t* step);
+template <typename R>
+void launch_adjoint_push(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,
+                         const RunK* runs, double dt, int32_t k0, const SettleRec* rec, const double* lambda,
+                         double* grad, unsigned long long* cnt);
 void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32);
 void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
@@ -129,6 +136,9 @@ struct start_ctx {
     unsigned long long* d_cnt = nullptr;
     uint64_t step = 0;
     uint64_t last_sort = 0;
+    // settle records for the adjoint (by load index), when recording
+    SettleRec* d_settle = nullptr;
+    bool record = false;
     // captured step chunks
     std::vector<GraphEntry> graphs;
     // timing
@@ -177,7 +187,8 @@ struct start_ctx {
         L.block = cfg.block == 128 ? 128 : 256;
         L.persistent = cfg.persistent ? 1 : 0;
         L.grid = n_cu * (1024 / L.block);  // 4 waves per SIMD x 4 SIMDs per CU
-        L.batch = runs.size() > 1 ? 1 : 0;
+        // per-lane run parameters unless the field is E_0 as given
+        L.batch = (runs.size() > 1 || d_E1 || runs[0].s0 != 1.0 || runs[0].s1 != 0.0) ? 1 : 0;
         return L;
     }
 };
@@ -238,7 +249,8 @@ void free_particles(start_ctx* c)
     drop_graphs(c);
     free_pbuf(c->A);
     free_pbuf(c->B);
-    void* ptrs[] = {c->d_wl, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp};
+    void* ptrs[] = {c->d_wl, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp, c->d_settle};
+    c->d_settle = nullptr;
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     c->d_wl = nullptr;
@@ -437,6 +449,7 @@ StepParams step_params(const start_ctx* c)
     S.nspecies = (int32_t)c->species.size();
     S.n_active = c->d_nactive;
     S.charge_len = c->charge_len();
+    S.settle = c->record ? c->d_settle : nullptr;
     return S;
 }
 
@@ -849,6 +862,10 @@ int start_load_batch(start_ctx* c, int32_t n_runs, const start_run_desc* runs, c
                                sizeof(unsigned long long) * c->n_runs() * std::max<int64_t>(c->hm.n_tris, 1),
                                c->stream));
     HIPCHECK(c, hipMemsetAsync(c->d_node_charge, 0, sizeof(double) * 2 * CL, c->stream));
+    if (c->record) {  // no particle has settled yet: step = -1 in every record
+        HIPCHECK(c, dalloc(&c->d_settle, n));
+        HIPCHECK(c, hipMemsetAsync(c->d_settle, 0xff, sizeof(SettleRec) * std::max<int64_t>(n, 1), c->stream));
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->cb = 0;
     c->latest = 0;
@@ -1121,6 +1138,79 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
     if (rc2 != START_OK) return rc2;
     if (best) *best = best_cfg;
     if (ms_per_step) *ms_per_step = best_ms;
+    return check_kernel_errors(c);
+}
+
+int start_record_settle(start_ctx* c, int32_t on)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured launches hold the record pointer
+    c->record = on != 0;
+    if (c->record && !c->d_settle && c->n > 0) {
+        HIPCHECK(c, dalloc(&c->d_settle, c->n));
+        HIPCHECK(c, hipMemset(c->d_settle, 0xff, sizeof(SettleRec) * c->n));
+    }
+    return START_OK;
+}
+
+int start_read_hits(start_ctx* c, double* hit_xyz, int32_t* hit_tri, int32_t* settle_step)
+{
+    if (!c || !hit_xyz || !hit_tri || !settle_step) return fail(c, START_ERR_ARG, "null argument");
+    if (c->n == 0) return START_OK;
+    if (!c->d_settle) return fail(c, START_ERR_STATE, "settle records are off (start_record_settle)");
+    const int64_t n = c->n;
+    double* dx = nullptr;
+    int32_t *dt = nullptr, *ds = nullptr;
+    hipError_t e = dalloc(&dx, 3 * n);
+    if (e == hipSuccess) e = dalloc(&dt, n);
+    if (e == hipSuccess) e = dalloc(&ds, n);
+    if (e == hipSuccess) {
+        if (c->mixed) launch_hit_points<float>(c->stream, c->mesh_view(), c->view<float>(), c->d_settle, dx, dt, ds);
+        else launch_hit_points<double>(c->stream, c->mesh_view(), c->view<double>(), c->d_settle, dx, dt, ds);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hit_xyz, dx, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hit_tri, dt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(settle_step, ds, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (dx) (void)hipFree(dx);
+    if (dt) (void)hipFree(dt);
+    if (ds) (void)hipFree(ds);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("hit points: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+int start_adjoint_push(start_ctx* c, const double* lambda, int32_t first_step, double* grad)
+{
+    if (!c || !lambda || !grad) return fail(c, START_ERR_ARG, "null argument");
+    if (c->n == 0) {
+        grad[0] = grad[1] = 0.0;
+        return START_OK;
+    }
+    if (!c->d_settle) return fail(c, START_ERR_STATE, "settle records are off (start_record_settle)");
+    if (c->prm.gas.model != START_MODEL_NONE)
+        return fail(c, START_ERR_STATE, "the adjoint covers the collisionless push (gas model NONE)");
+    const int64_t n = c->n;
+    double *dl = nullptr, *dg = nullptr;
+    hipError_t e = dalloc(&dl, 3 * n);
+    if (e == hipSuccess) e = dalloc(&dg, 2);
+    if (e == hipSuccess) e = hipMemcpyAsync(dl, lambda, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(dg, 0, sizeof(double) * 2, c->stream);
+    if (e == hipSuccess) {
+        if (c->mixed)
+            launch_adjoint_push<float>(c->stream, c->mesh_view(), c->view<float>(), c->d_species, c->d_runs,
+                                       c->prm.dt, first_step, c->d_settle, dl, dg, c->d_cnt);
+        else
+            launch_adjoint_push<double>(c->stream, c->mesh_view(), c->view<double>(), c->d_species, c->d_runs,
+                                        c->prm.dt, first_step, c->d_settle, dl, dg, c->d_cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(grad, dg, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (dl) (void)hipFree(dl);
+    if (dg) (void)hipFree(dg);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("adjoint: ") + hipGetErrorString(e));
     return check_kernel_errors(c);
 }
 

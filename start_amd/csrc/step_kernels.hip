@@ -608,6 +608,19 @@ __device__ inline int classify(const double lam[4], double tau, int& fneg)
 
 enum StepOutcome { kDone = 0, kSettledNow = 1, kDeferNow = 2, kFrozenNow = 3 };
 
+// settle record for the adjoint (rare; out of line, scalar arguments)
+__device__ __noinline__ void record_settle(SettleRec* rec, uint32_t pidx, uint32_t step, double x0, double y0,
+                                           double z0)
+{
+    SettleRec r;
+    r.x0 = x0;
+    r.y0 = y0;
+    r.z0 = z0;
+    r.step = (int32_t)step;
+    r.pad = 0;
+    rec[pidx] = r;
+}
+
 template <typename R>
 struct PState {
     R x, y, z, vx, vy, vz;
@@ -814,6 +827,11 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
                     P.hit[i] = r.hit_tri;
                     atomicAdd(&tri_count[rk->tri_base + r.hit_tri], 1ull);
                     atomicAdd(&cnt[kCntSettled], 1ull);
+                    if (S.settle) {
+                        double a[3];
+                        abs_pos(M, t_before, px, py, pz, a);
+                        record_settle(S.settle, P.pidx[i], step0 + (uint32_t)ks, a[0], a[1], a[2]);
+                    }
                     break;
                 }
                 if (r.outcome == kDeferNow) {
@@ -990,6 +1008,8 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
         int32_t hit_tri = -1;
         double x0[3] = {it.x0, it.y0, it.z0};
         bool settled = finish_exact(M, S, k, x0, s, sink, hit_tri, exh, err);
+        uint32_t settle_step = step0 + (uint32_t)it.k;  // step and start of the settling segment
+        double sx = x0[0], sy = x0[1], sz = x0[2];
         if (!settled) {
             const uint32_t vid = S.model != 0 ? P.pidx[i] - rk->first : 0u;
             unsigned long long extra = 0;
@@ -999,15 +1019,20 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
                 StepResult rr = step_one<R, true>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (rr.outcome == kFrozenNow) break;
                 ++extra;
-                if (rr.outcome == kDeferNow) {
-                    ++nexact;
+                if (rr.outcome == kDeferNow || rr.outcome == kSettledNow) {
                     double x0s[3];
                     abs_pos(M, t_before, px, py, pz, x0s);
-                    settled = finish_exact(M, S, k, x0s, s, sink, hit_tri, exh, err);
-                }
-                if (rr.outcome == kSettledNow) {
-                    hit_tri = rr.hit_tri;
-                    settled = true;
+                    if (rr.outcome == kDeferNow) {
+                        ++nexact;
+                        settled = finish_exact(M, S, k, x0s, s, sink, hit_tri, exh, err);
+                    } else {
+                        hit_tri = rr.hit_tri;
+                        settled = true;
+                    }
+                    settle_step = step0 + (uint32_t)ks;
+                    sx = x0s[0];
+                    sy = x0s[1];
+                    sz = x0s[2];
                 }
                 if (settled) break;
             }
@@ -1018,6 +1043,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
             P.hit[i] = hit_tri;
             atomicAdd(&tri_count[rk->tri_base + hit_tri], 1ull);
             atomicAdd(&cnt[kCntSettled], 1ull);
+            if (S.settle) record_settle(S.settle, P.pidx[i], settle_step, sx, sy, sz);
         }
         sink.flush(M);
         P.x[i] = s.x;
@@ -1182,6 +1208,136 @@ __global__ void abs_positions_kernel(DevMesh M, const float* __restrict__ lx, co
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint of the collisionless push and the surface-hit map.
+//
+// Forward, step k with owner t_k of x_k:
+//   v_{k+1} = v_k + a(t_k) dt,  a = (q/m) (s0 E_0 + s1 E_1),  x_{k+1} = x_k + v_{k+1} dt.
+// The particle settles in step k* on triangle T (plane n.x = n.A); its hit
+// point is x_h = x_a + s D with x_a = x_{k*}, D = x_{k*+1} - x_a and
+// s = n.(A - x_a) / n.D.  Linearised: dx_h = P (dx_a + s dt dv_{k*+1}) with
+// P = I - D n^T / (n.D).  For a loss L with lambda = dL/dx_h the reverse sweep is
+//   lx = P^T lambda                    adjoint of every x_k (E is constant per tet)
+//   mu_k* = s dt P^T lambda            adjoint of v_{k*+1}
+//   mu_{k-1} = mu_k + dt lx
+//   dL/ds_j += mu_k . (q/m) E_j(t_k) dt,   k = k*, ..., k0.
+// The owners t_k are recomputed backwards -- x_{k-1} = x_k - v_k dt with v_k
+// the push of t_k undone -- instead of storing the trajectory.
+// ---------------------------------------------------------------------------
+__device__ inline void hit_plane(const DevMesh& M, int32_t tri, const double* xa, const double* D, double* n,
+                                 double& s)
+{
+    double A[3], B[3], C[3];
+    load_node(M, M.tris[3 * (int64_t)tri], A);
+    load_node(M, M.tris[3 * (int64_t)tri + 1], B);
+    load_node(M, M.tris[3 * (int64_t)tri + 2], C);
+    double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    double w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    n[0] = u[1] * w[2] - u[2] * w[1];
+    n[1] = u[2] * w[0] - u[0] * w[2];
+    n[2] = u[0] * w[1] - u[1] * w[0];
+    double nD = (n[0] * D[0] + n[1] * D[1]) + n[2] * D[2];
+    double nA = (n[0] * (A[0] - xa[0]) + n[1] * (A[1] - xa[1])) + n[2] * (A[2] - xa[2]);
+    s = nA / nD;
+}
+
+// hit points, triangles and settle steps of the settled particles, in load order
+template <typename R>
+__global__ void hit_points_kernel(DevMesh M, DevParticles<R> P, const SettleRec* __restrict__ rec,
+                                  double* __restrict__ xh, int32_t* __restrict__ tri, int32_t* __restrict__ step)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    uint32_t p = P.pidx[i];
+    SettleRec r = rec[p];
+    if (!(P.flags[i] & kSettled) || r.step < 0) {
+        xh[3 * (int64_t)p] = xh[3 * (int64_t)p + 1] = xh[3 * (int64_t)p + 2] = 0.0;
+        tri[p] = -1;
+        step[p] = -1;
+        return;
+    }
+    double xb[3], xa[3] = {r.x0, r.y0, r.z0}, D[3], n[3], s;
+    abs_pos(M, P.tet[i], P.x[i], P.y[i], P.z[i], xb);
+    for (int k = 0; k < 3; ++k) D[k] = xb[k] - xa[k];
+    hit_plane(M, P.hit[i], xa, D, n, s);
+    for (int k = 0; k < 3; ++k) xh[3 * (int64_t)p + k] = xa[k] + s * D[k];
+    tri[p] = P.hit[i];
+    step[p] = r.step;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) adjoint_push_kernel(DevMesh M, DevParticles<R> P,
+                                                           const SpeciesK* __restrict__ sp,
+                                                           const RunK* __restrict__ runs, double dt, int32_t k0,
+                                                           const SettleRec* __restrict__ rec,
+                                                           const double* __restrict__ lambda,
+                                                           double* __restrict__ grad,
+                                                           unsigned long long* __restrict__ cnt)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double g0 = 0.0, g1 = 0.0;
+    int err = 0;
+    if (i < P.n) {
+        const int32_t f = P.flags[i];
+        const uint32_t p = P.pidx[i];
+        const double lam[3] = {lambda[3 * (int64_t)p], lambda[3 * (int64_t)p + 1], lambda[3 * (int64_t)p + 2]};
+        const SettleRec r = rec[p];
+        if ((f & kSettled) && r.step >= k0 && (lam[0] != 0.0 || lam[1] != 0.0 || lam[2] != 0.0)) {
+            const SpeciesK& k = sp[(f >> kSpeciesShift) & 0xff];
+            const RunK& rk = runs[(uint32_t)f >> kRunShift];
+            const double qm = k.charge / k.mass;
+            int32_t t = P.tet[i];
+            double xb[3], x[3] = {r.x0, r.y0, r.z0}, D[3], n[3], s;
+            abs_pos(M, t, P.x[i], P.y[i], P.z[i], xb);
+            for (int c = 0; c < 3; ++c) D[c] = xb[c] - x[c];
+            hit_plane(M, P.hit[i], x, D, n, s);
+            const double nD = (n[0] * D[0] + n[1] * D[1]) + n[2] * D[2];
+            const double Dl = (D[0] * lam[0] + D[1] * lam[1]) + D[2] * lam[2];
+            double pl[3], mu[3];
+            for (int c = 0; c < 3; ++c) {
+                pl[c] = lam[c] - n[c] * (Dl / nD);
+                mu[c] = (s * dt) * pl[c];
+            }
+            double v[3] = {(double)P.vx[i], (double)P.vy[i], (double)P.vz[i]};
+            for (int32_t kk = r.step;; --kk) {
+                double e0[3] = {0.0, 0.0, 0.0}, e1[3] = {0.0, 0.0, 0.0};
+                if (t >= 0) {
+                    for (int c = 0; c < 3; ++c) e0[c] = M.E[3 * (int64_t)t + c];
+                    if (M.E1)
+                        for (int c = 0; c < 3; ++c) e1[c] = M.E1[3 * (int64_t)t + c];
+                }
+                g0 += (((mu[0] * e0[0] + mu[1] * e0[1]) + mu[2] * e0[2]) * qm) * dt;
+                g1 += (((mu[0] * e1[0] + mu[1] * e1[1]) + mu[2] * e1[2]) * qm) * dt;
+                if (kk <= k0) break;
+                if (t >= 0) {  // undo the push of step kk: forward (v + h) + h
+                    double E[3];
+                    for (int c = 0; c < 3; ++c) E[c] = rk.s0 * e0[c] + rk.s1 * e1[c];
+                    if (!(E[0] == 0.0 && E[1] == 0.0 && E[2] == 0.0)) {
+                        for (int c = 0; c < 3; ++c) {
+                            double h = (((E[c] * k.charge) / k.mass) * dt) / 2.0;
+                            v[c] = (v[c] - h) - h;
+                        }
+                    }
+                }
+                for (int c = 0; c < 3; ++c) x[c] = x[c] - v[c] * dt;
+                int strict = 0;
+                t = t >= 0 ? locate_near(M, t, x, &strict, &err) : locate_grid(M, x, &strict, &err);
+                for (int c = 0; c < 3; ++c) mu[c] = mu[c] + dt * pl[c];
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        g0 += __shfl_xor(g0, off);
+        g1 += __shfl_xor(g1, off);
+    }
+    if ((threadIdx.x & 63) == 0 && (g0 != 0.0 || g1 != 0.0)) {
+        unsafeAtomicAdd(&grad[0], g0);
+        unsafeAtomicAdd(&grad[1], g1);
+    }
+    if (err) atomicOr(&cnt[kCntError], 1ull);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -1231,6 +1387,32 @@ void launch_gather(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, co
 {
     if (P.n > 0) gather_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, sp, runs, node_charge);
 }
+
+template <typename R>
+void launch_hit_points(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SettleRec* rec, double* xh,
+                       int32_t* tri, int32_t* step)
+{
+    if (P.n > 0) hit_points_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, rec, xh, tri, step);
+}
+
+template <typename R>
+void launch_adjoint_push(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,
+                         const RunK* runs, double dt, int32_t k0, const SettleRec* rec, const double* lambda,
+                         double* grad, unsigned long long* cnt)
+{
+    if (P.n > 0)
+        adjoint_push_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, sp, runs, dt, k0, rec, lambda, grad, cnt);
+}
+template void launch_hit_points<double>(hipStream_t, const DevMesh&, const DevParticles<double>&, const SettleRec*,
+                                        double*, int32_t*, int32_t*);
+template void launch_hit_points<float>(hipStream_t, const DevMesh&, const DevParticles<float>&, const SettleRec*,
+                                       double*, int32_t*, int32_t*);
+template void launch_adjoint_push<double>(hipStream_t, const DevMesh&, const DevParticles<double>&, const SpeciesK*,
+                                          const RunK*, double, int32_t, const SettleRec*, const double*, double*,
+                                          unsigned long long*);
+template void launch_adjoint_push<float>(hipStream_t, const DevMesh&, const DevParticles<float>&, const SpeciesK*,
+                                         const RunK*, double, int32_t, const SettleRec*, const double*, double*,
+                                         unsigned long long*);
 
 template <typename R>
 void launch_node_density(hipStream_t s, const DevParticles<R>& P, const SpeciesK* sp, int32_t mode,

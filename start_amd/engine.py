@@ -152,6 +152,24 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_autotune(self.ctx, int(trial_steps), C.byref(cfg), C.byref(ms)))
         return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_ if k != "pad"}, ms.value
 
+    def record_settle(self, on: bool = True):
+        self._check(_lib.lib().start_record_settle(self.ctx, int(on)))
+
+    def hits(self):
+        """Hit point, triangle and settle step of every particle (load order; -1 if not settled)."""
+        xh = np.empty((self.n, 3), dtype=np.float64)
+        tri = np.empty(self.n, dtype=np.int32)
+        step = np.empty(self.n, dtype=np.int32)
+        self._check(_lib.lib().start_read_hits(self.ctx, _dp(xh), _ip(tri), _ip(step)))
+        return xh, tri, step
+
+    def adjoint_push(self, lam: np.ndarray, first_step: int = 0) -> np.ndarray:
+        """dL/d(field scales) for lam = dL/d(hit point) per particle (n x 3, load order)."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+        g = np.zeros(2, dtype=np.float64)
+        self._check(_lib.lib().start_adjoint_push(self.ctx, _dp(lam), int(first_step), _dp(g)))
+        return g
+
     def set_tet_field(self, E: np.ndarray):
         E = np.ascontiguousarray(E, dtype=np.float64).reshape(-1)
         self._check(_lib.lib().start_set_tet_field(self.ctx, _dp(E)))

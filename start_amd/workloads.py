@@ -82,6 +82,53 @@ def large_box(n_particles: int = 1_000_000, steps: int = 2000, cells: int = 50,
 
 
 @dataclass
+class Plates:
+    """Two parallel electrode plates with particles emitted from the top one;
+    the side faces of the gap are open (particles leave there)."""
+    name: str
+    mesh: meshgen.TetMesh
+    species: list
+    gas: Gas
+    dt: float
+    steps: int
+    seed: int
+    pos: np.ndarray
+    vel: np.ndarray
+    sp_idx: np.ndarray
+    phi_top: np.ndarray     # top plate at 1 V, bottom at 0 V (field basis 0)
+    phi_bottom: np.ndarray  # bottom plate at 1 V, top at 0 V (field basis 1)
+    bottom: np.ndarray      # bool per surface triangle: on the bottom plate
+
+
+def plates(n_particles: int = 262_144, nx: int = 51, ny: int = 51, nz: int = 13, steps: int = 1000,
+           dt: float = 1e-8, seed: int = 1003, side: float = 0.2, gap: float = 0.05) -> Plates:
+    """Plates side x side at a gap (51 x 51 x 13 hexes x 6 = 202,878 tets); Cu+
+    (+1e) with lognormal energies (mean 10 eV, sigma_ln 0.5) and cosine-law
+    directions into the gap from points uniform on the top plate, 0.1 mm below
+    it; no gas."""
+    m = meshgen.box_kuhn(nx, ny, nz, side, side, gap)
+    z = m.nodes[m.tris][:, :, 2]
+    bottom_all = np.all(z == 0.0, axis=1)
+    on_plate = bottom_all | np.all(z == gap, axis=1)
+    mesh = meshgen.TetMesh(m.nodes, m.tets, np.ascontiguousarray(m.tris[on_plate]))
+    rng = np.random.default_rng(seed)
+    cu = species("Cu", "Ar", charge_units=1.0)
+    g = gas("Ar", 0.0, 300.0, "NONE")
+    mu = np.log(10.0) - 0.5 * 0.5 ** 2
+    energy = np.exp(rng.normal(mu, 0.5, n_particles))
+    speed = np.sqrt(2.0 * energy * EV_J / MASS["Cu"])
+    ct = np.sqrt(rng.uniform(0.0, 1.0, n_particles))
+    st = np.sqrt(1.0 - ct * ct)
+    ph = rng.uniform(0.0, 2.0 * np.pi, n_particles)
+    vel = speed[:, None] * np.stack([st * np.cos(ph), st * np.sin(ph), -ct], axis=1)
+    pos = np.stack([rng.uniform(0.0, side, n_particles), rng.uniform(0.0, side, n_particles),
+                    np.full(n_particles, gap - 1e-4)], axis=1)
+    return Plates("plates", mesh, [cu], g, dt, steps, seed, pos, vel, np.zeros(n_particles, dtype=np.int32),
+                  meshgen.linear_potential(mesh, 0.0, 1.0), meshgen.linear_potential(mesh, 1.0, 0.0),
+                  bottom_all[on_plate])
+
+
+@dataclass
 class Sweep:
     """Independent runs on one mesh: particle counts, electrode voltages and
     pressures per run, rows of the runs consecutive in run order."""

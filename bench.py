@@ -157,6 +157,40 @@ def sweep_run(steps: int, warmup: int, trial_steps: int = 200):
     return out
 
 
+def adjoint_run(reps: int = 3):
+    """Forward (collisionless push, surface-hit map) and reverse sweep on the
+    parallel-plate workload: 262,144 Cu+, 202,878 tets, 1,000 steps."""
+    from start_amd import workloads
+    from start_amd.adjoint import PlateDeposition, moments_lambda
+    w = workloads.plates()
+    model = PlateDeposition(w)
+    V = np.array([150.0, 30.0])
+    model.forward(V)  # warm-up
+    fwd, bwd = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        xh, tri, step = model.forward(V)
+        t1 = time.perf_counter()
+        sel = model.on_target(tri)
+        lam = moments_lambda(xh, sel, np.array([0.0, 0.0, 1.0, 1.0]))
+        t2 = time.perf_counter()
+        g = model.backward(lam)
+        t3 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t3 - t2)
+    stats = model.eng.stats()
+    reverse_steps = int((step[sel] + 1).sum())
+    model.close()
+    f, b = min(fwd), min(bwd)
+    return {"workload": "plates: 262,144 Cu+ (lognormal 10 eV, cosine law), 202,878 tets, 1,000 steps of 1e-8 s, "
+                        "V = (150, 30) V",
+            "forward_s": round(f, 5), "forward_particle_steps_per_s": round(stats["processed"] / f, 1),
+            "backward_s": round(b, 5), "backward_particle_steps_per_s": round(reverse_steps / b, 1),
+            "on_target": int(sel.sum()), "grad": [float(g[0]), float(g[1])],
+            "note": "forward includes the particle upload and owner search; backward includes the "
+                    "lambda upload; best of %d" % reps}
+
+
 def roofline(r, mode):
     prec = MODES[mode][0] if mode in MODES else r["precision"]
     B = BYTES_PER_PS[prec]
@@ -219,6 +253,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep-steps", type=int, default=500,
                     help="timed steps of the batched sweep (64 runs); 0 skips it")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES, 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -252,6 +287,10 @@ def main():
     sweep = None
     if args.sweep_steps > 0:
         sweep = sweep_run(args.sweep_steps, args.warmup)
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_run()
+        log(f"[bench] adjoint: {adj}")
     head = HEADLINE if HEADLINE in results else modes[0]
     r = results[head]
 
@@ -304,6 +343,8 @@ def main():
         }
         if "fp64_per_step" in results:
             out["sweep"]["mixed_vs_fp64_per_step"] = round(m["ps_per_s"] / results["fp64_per_step"]["ps_per_s"], 3)
+    if adj:
+        out["adjoint"] = adj
     print(json.dumps(out), flush=True)
 
 

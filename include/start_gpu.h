@@ -202,8 +202,17 @@ int start_set_time_step(start_ctx* ctx, double dt);
 /* Per-tet field E (3*n_tets), as ChargeDensityEquationSolver::solve leaves it
  * (ChargeDensityEquationSolver.cpp:42, VolumetricMesh.cpp:382-411). */
 int start_set_tet_field(start_ctx* ctx, const double* E);
-/* Node potentials (n_nodes) -> per-tet E = -grad(phi) on the device. */
+/* Node potentials (n_nodes) -> per-tet E = -grad(phi) on the device
+ * (the FEM solve's hand-off, MatrixEquationSolver.cpp:53-64 then
+ * VolumetricMesh.cpp:382-411).  Asynchronous and stream-ordered: the array is
+ * copied into pinned staging before the call returns, steps already queued
+ * keep the previous field and later steps see the new one, so a field refresh
+ * every few steps costs no device synchronisation. */
 int start_set_node_potential(start_ctx* ctx, const double* phi);
+/* Reads back per-tet field basis `which` (3*n_tets, FP64) as the step kernels
+ * see it, e.g. after start_set_node_potential: the field side of the hand-off
+ * (VolumetricMesh.cpp:382-411 stores E per tet; no reference accessor). */
+int start_read_tet_field(start_ctx* ctx, int32_t which, double* E);
 
 /* ParticleDynamicsProcessor::process, nsteps times (ParticleDynamicsProcessor.cpp:213-277),
  * each step followed by the relocation and charge the next gather needs

@@ -27,6 +27,7 @@ EXPORTS = [
     "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
     "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
+    "start_read_tet_field",
 ]
 DENSITY = {"barycentric": 0, "tet_average": 1}
 
@@ -110,6 +111,7 @@ def lib() -> C.CDLL:
     L.start_set_time_step.argtypes = [vp, d]
     L.start_load_batch.argtypes = [vp, i32, P(RunDesc)] + [P(d)] * 6 + [P(i32), P(u64)]
     L.start_set_tet_field_basis.argtypes = [vp, i32, P(d)]
+    L.start_read_tet_field.argtypes = [vp, i32, P(d)]
     L.start_set_node_potential_basis.argtypes = [vp, i32, P(d)]
     L.start_read_node_charge_runs.argtypes = [vp, P(d)]
     L.start_read_tri_counts_runs.argtypes = [vp, P(u64)]

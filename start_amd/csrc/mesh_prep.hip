@@ -193,11 +193,15 @@ std::string prepare_mesh(HostMesh& m, int64_t n_nodes, const double* nodes, int6
             }
             rowsum = std::max(rowsum, s);
         }
-        // |computed - exact| of lambda at points with ||lambda||_1 <= ~1:
-        //   map rounding (eps * rowsum * diam) + (x - v0) rounding (eps * rowsum * diam)
-        //   + 3-term dot product (3 eps * rowsum * diam) + lambda_0 sum (4 eps)
-        // taken with a factor 4 margin.
-        double bound = 4.0 * kEps * (6.0 * rowsum * diam + 8.0);
+        // |computed - exact| of lambda at a point x with |x - v0|_inf <= D, to
+        // first order in eps: map rounding (eps rowsum D) + (x - v0) rounding
+        // (eps rowsum D) + 3-term dot product (3 eps rowsum D) = 5 eps rowsum D
+        // for each of lambda_1..3; lambda_0 = 1 - ((l1 + l2) + l3) adds the
+        // three and 3 eps (1 + sum |lambda_j|) <= 9 eps.  So <= 15 eps R + 9 eps
+        // with R = rowsum D.  The step kernels classify only points with
+        // sum |lambda| <= 2, hence |x - v0|_inf <= 2 diam (+ O(tau)): D = 2 diam,
+        // and tau = 4 eps (6 R + 8) is 1.6x the bound.
+        double bound = 4.0 * kEps * (12.0 * rowsum * diam + 8.0);
         tau = std::max(tau, bound);
     }
     if (tau > 1e-6) return "mesh too ill-conditioned for the certified barycentric fast path";

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstring>
 #include <new>
 #include <string>
 #include <vector>
@@ -105,6 +106,12 @@ struct start_ctx {
     float* d_E32 = nullptr;
     double* d_E1 = nullptr;  // field basis 1 (allocated on first use)
     float* d_E1_32 = nullptr;
+    // node potential hand-off: device copy, pinned staging buffer, and the
+    // event after which the staging buffer may be rewritten
+    double* d_phi = nullptr;
+    double* h_phi = nullptr;
+    hipEvent_t phi_copied = nullptr;
+    bool phi_pending = false;
     int4* d_tnode = nullptr;
     int4* d_tnbr = nullptr;
     int32_t* d_tris = nullptr;
@@ -301,11 +308,13 @@ SpeciesK species_constants(const start_species_desc& s, const start_gas_desc& g)
 }
 
 // FP32 copy of the barycentric maps (A row-major, then v0) and the FP32
-// certification threshold.  For a stored local position l (|l|_inf <= diam):
+// certification threshold.  For a stored local position l (|l|_inf <= D):
 // lambda_j = a_j . l with a_j rounded to FP32 and a 3-term FP32 dot product
-// errs by <= (eps32 + gamma_3) |A|_inf diam ~ 4 eps32 R (R = |A|_inf diam);
+// errs by <= (eps32 + gamma_3) |A|_inf D ~ 4 eps32 R (R = |A|_inf D);
 // lambda_0 = 1 - ((l1 + l2) + l3) adds the three and gamma_3 (1 + sum|l_j|),
-// so |lambda32 - lambda| <= (12 R + 7) eps32; tau32 takes 1.25x that.
+// so |lambda32 - lambda| <= (12 R + 7) eps32; tau32 takes 1.25x that at
+// D = 2 diam (the step kernels classify only points with sum |lambda| <= 2,
+// which lie within 2 diam of v0).
 float build_geo32(const HostMesh& hm, std::vector<float>& g32)
 {
     const double eps32 = 5.9604644775390625e-08;  // 2^-24
@@ -332,7 +341,7 @@ float build_geo32(const HostMesh& hm, std::vector<float>& g32)
             for (int b = a + 1; b < 4; ++b)
                 for (int k = 0; k < 3; ++k)
                     diam = std::max(diam, fabs(hm.nodes[3 * (size_t)v[a] + k] - hm.nodes[3 * (size_t)v[b] + k]));
-        tau = std::max(tau, 1.25 * eps32 * (12.0 * rowsum * diam + 7.0));
+        tau = std::max(tau, 1.25 * eps32 * (12.0 * rowsum * 2.0 * diam + 7.0));
     }
     return (float)tau;
 }
@@ -777,9 +786,11 @@ void start_destroy(start_ctx* c)
                     c->d_E1_32,     c->d_tnode,     c->d_tnbr,        c->d_tris,      c->d_tri_deg,  c->d_tgs,
                     c->d_tgl,       c->d_sgs,       c->d_sgl,         c->d_tri_count, c->d_node_charge,
                     c->d_species,   c->d_cnt,       c->d_nactive,     c->d_runs,      c->d_tet_vol,  c->d_node_vol,
-                    c->d_tet_q,     c->d_adj_start, c->d_adj_list,    c->d_tet_n};
+                    c->d_tet_q,     c->d_adj_start, c->d_adj_list,    c->d_tet_n,     c->d_phi};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (c->h_phi) (void)hipHostFree(c->h_phi);
+    if (c->phi_copied) (void)hipEventDestroy(c->phi_copied);
     for (auto& e : c->events) {
         (void)hipEventDestroy(e.a0);
         (void)hipEventDestroy(e.a1);
@@ -1000,6 +1011,17 @@ int start_set_tet_field_basis(start_ctx* c, int32_t which, const double* E)
 
 int start_set_tet_field(start_ctx* c, const double* E) { return start_set_tet_field_basis(c, 0, E); }
 
+int start_read_tet_field(start_ctx* c, int32_t which, double* E)
+{
+    if (!c || !E) return fail(c, START_ERR_ARG, "null argument");
+    if (which != 0 && which != 1) return fail(c, START_ERR_ARG, "field basis must be 0 or 1");
+    const double* dE = which ? c->d_E1 : c->d_E;
+    if (!dE) return fail(c, START_ERR_STATE, "field basis 1 was never set");
+    HIPCHECK(c, hipMemcpyAsync(E, dE, sizeof(double) * 3 * (size_t)c->hm.n_tets, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return START_OK;
+}
+
 int start_set_node_potential_basis(start_ctx* c, int32_t which, const double* phi)
 {
     if (!c || !phi) return fail(c, START_ERR_ARG, "null argument");
@@ -1011,15 +1033,26 @@ int start_set_node_potential_basis(start_ctx* c, int32_t which, const double* ph
         if (c->mixed) HIPCHECK(c, dalloc(&c->d_E1_32, 4 * T));
         drop_graphs(c);
     }
-    double* d_phi = nullptr;
-    HIPCHECK(c, dalloc(&d_phi, c->hm.n_nodes));
-    hipError_t e = hipMemcpyAsync(d_phi, phi, sizeof(double) * c->hm.n_nodes, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) {
-        launch_field(c->stream, c->mesh_view(), d_phi, which ? c->d_E1 : c->d_E, which ? c->d_E1_32 : c->d_E32);
-        e = hipGetLastError();
+    // Stream-ordered and asynchronous: launches already queued use the field
+    // they were queued with, later ones the new one.  The caller's array is
+    // copied into pinned staging before the call returns; the only wait is for
+    // the previous call's copy out of that staging buffer.
+    const size_t bytes = sizeof(double) * (size_t)c->hm.n_nodes;
+    if (!c->d_phi) {
+        HIPCHECK(c, dalloc(&c->d_phi, c->hm.n_nodes));
+        HIPCHECK(c, hipHostMalloc((void**)&c->h_phi, std::max<size_t>(bytes, 8), hipHostMallocDefault));
+        HIPCHECK(c, hipEventCreateWithFlags(&c->phi_copied, hipEventDisableTiming));
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(d_phi);
+    if (c->phi_pending) {
+        HIPCHECK(c, hipEventSynchronize(c->phi_copied));
+        c->phi_pending = false;
+    }
+    std::memcpy(c->h_phi, phi, bytes);
+    HIPCHECK(c, hipMemcpyAsync(c->d_phi, c->h_phi, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipEventRecord(c->phi_copied, c->stream));
+    c->phi_pending = true;
+    launch_field(c->stream, c->mesh_view(), c->d_phi, which ? c->d_E1 : c->d_E, which ? c->d_E1_32 : c->d_E32);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("field from potential: ") + hipGetErrorString(e));
     return START_OK;
 }

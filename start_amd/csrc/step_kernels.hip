@@ -587,7 +587,11 @@ __device__ __noinline__ V3<R> collide_velocity(int model, double mass_cp, double
 
 // Barycentric classification with a certified threshold: strictly inside,
 // strictly outside exactly one face (strictly inside the other three), or
-// uncertain.
+// uncertain.  tau bounds the rounding of lambda at points within 2 diam of v0
+// (diam: largest vertex coordinate difference).  Since x - v0 = sum_j
+// lambda_j (v_j - v0), |x - v0|_inf <= diam * sum |lambda_j|, so the
+// classification is only trusted when sum |lambda| <= 2: a long step that
+// ends far outside the tet is left to the exact walk.
 enum { kInside = 0, kOneFace = 1, kUncertain = 2 };
 __device__ inline int classify(const double lam[4], double tau, int& fneg)
 {
@@ -601,6 +605,7 @@ __device__ inline int classify(const double lam[4], double tau, int& fneg)
             fneg = q;
         }
     }
+    if (fabs(lam[0]) + fabs(lam[1]) + fabs(lam[2]) + fabs(lam[3]) > 2.0) return kUncertain;
     if (npos == 4) return kInside;
     if (npos == 3 && nneg == 1) return kOneFace;
     return kUncertain;
@@ -1371,7 +1376,9 @@ void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, cons
         }
     }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
-    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 64u));
+    // the work list is only known on the device: a grid that covers up to
+    // 65,536 deferred particles at once, the rest by grid stride
+    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 1024u));
     step_resolve<R><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
 }
 

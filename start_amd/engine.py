@@ -175,8 +175,15 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_set_tet_field(self.ctx, _dp(E)))
 
     def set_node_potential(self, phi: np.ndarray):
+        """Node potentials -> per-tet E on the device (asynchronous, stream-ordered)."""
         phi = np.ascontiguousarray(phi, dtype=np.float64)
         self._check(_lib.lib().start_set_node_potential(self.ctx, _dp(phi)))
+
+    def tet_field(self, which: int = 0) -> np.ndarray:
+        """The per-tet field basis `which` as the step kernels see it, (n_tets, 3)."""
+        out = np.empty((self.mesh.n_tets, 3), dtype=np.float64)
+        self._check(_lib.lib().start_read_tet_field(self.ctx, int(which), _dp(out)))
+        return out
 
     # --- the two step-loop calls ------------------------------------------------
     def process(self, nsteps: int = 1):

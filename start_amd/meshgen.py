@@ -100,6 +100,67 @@ def boundary_faces(tets: np.ndarray) -> np.ndarray:
     return faces[once]
 
 
+def concentric_disk(a: np.ndarray, b: np.ndarray):
+    """Shirley-Chiu concentric map of the square [-1, 1]^2 onto the unit disk:
+    area preserving (Jacobian pi/4 everywhere), orientation preserving, and
+    square rings go to circles."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    use_a = np.abs(a) > np.abs(b)
+    r = np.where(use_a, a, b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        phi = np.where(use_a, (np.pi / 4.0) * (b / a), np.pi / 2.0 - (np.pi / 4.0) * (a / b))
+    phi = np.where(r == 0.0, 0.0, phi)
+    return r * np.cos(phi), r * np.sin(phi)
+
+
+@dataclass
+class Chamber:
+    mesh: TetMesh        # tris: the boundary faces, then the substrate triangles
+    substrate: np.ndarray  # bool per triangle
+    target: np.ndarray     # bool per triangle: boundary faces on the target disk
+    radius: float
+    height: float
+
+
+def cylinder_chamber(n: int = 80, nz: int = 78, radius: float = 0.15, height: float = 0.3,
+                     r_target: float = 0.05, z_substrate: float = 0.1, r_substrate: float = 0.1325) -> Chamber:
+    """Sputtering chamber: a cylinder (radius, height) of n x n x nz hexes x 6
+    tets (80 x 80 x 78 gives 2,995,200 tets and 518,319 nodes) mapped from a box
+    by the concentric map.  Surface triangles: every boundary face (walls, the
+    top with the target disk r <= r_target, the floor), then the internal faces
+    of the layer z = z_substrate within r_substrate (the substrate plate; about
+    10,000 triangles at the default sizes).  z_substrate must be a layer of the
+    grid (height * k / nz)."""
+    box = box_kuhn(n, n, nz, 2.0, 2.0, height, origin=(-1.0, -1.0, 0.0))
+    x, y = concentric_disk(box.nodes[:, 0], box.nodes[:, 1])
+    nodes = np.stack([radius * x, radius * y, box.nodes[:, 2]], axis=1)
+    tets = _orient_positive(nodes, box.tets.astype(np.int64))
+    a, b, c, d = (nodes[tets[:, q]] for q in range(4))
+    vol = np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a))
+    if not (vol > 0).all():
+        raise ValueError("concentric map produced a degenerate tet")
+    walls = boundary_faces(tets)
+    k = int(round(z_substrate / height * nz))
+    z_layer = box.nodes[:, 2][((k * (n + 1)) * (n + 1))]  # the grid's own value of that layer
+    faces = tets[:, FACES].reshape(-1, 3)
+    on_layer = np.all(nodes[faces][:, :, 2] == z_layer, axis=1)
+    cen = nodes[faces[on_layer]].mean(axis=1)
+    inside = np.hypot(cen[:, 0], cen[:, 1]) <= r_substrate
+    sub = faces[on_layer][inside]
+    key = np.sort(sub, axis=1)
+    _, first = np.unique(key, axis=0, return_index=True)
+    sub = sub[np.sort(first)]
+    tris = np.concatenate([walls, sub]).astype(np.int32)
+    wc = nodes[walls].mean(axis=1)
+    target = np.zeros(len(tris), dtype=bool)
+    target[: len(walls)] = (np.all(nodes[walls][:, :, 2] == height, axis=1)
+                            & (np.hypot(wc[:, 0], wc[:, 1]) <= r_target))
+    substrate = np.zeros(len(tris), dtype=bool)
+    substrate[len(walls):] = True
+    return Chamber(TetMesh(nodes, tets.astype(np.int32), tris), substrate, target, radius, height)
+
+
 def linear_potential(mesh: TetMesh, v_bottom: float, v_top: float, axis: int = 2) -> np.ndarray:
     """Node potential of two Dirichlet plates at the min/max of `axis`."""
     c = mesh.nodes[:, axis]

@@ -9,6 +9,12 @@ large_box  0.5 m cube, 50^3 hexes x 6 tets (750,000 tets, 132,651 nodes, 30,000
            triangles); 1,000,000 Ar+ (+1e), 1 eV Maxwellian, uniform positions;
            VHS with Ar at 1 Pa / 300 K, d_ref 4.17e-10 m, omega 0.81f; charge
            scattered every step; dt 1e-9 s, 2,000 steps; potential 0 / 500 V.
+chamber    cylinder r 0.15 m, h 0.3 m (80 x 80 x 78 hexes x 6 tets: 2,995,200
+           tets, 518,319 nodes), target disk r <= 0.05 m on top, substrate plate
+           (10,080 triangles) at z = 0.1 m; 16,000,000 particles: Ti neutrals
+           (Thompson, U_b 4.89 eV, cosine law) and Ar+ (1 eV Maxwellian); VSS Ar
+           at 0.3 Pa; node potential refreshed every 10 steps; dt 5e-8 s,
+           1,000 steps; mixed precision.
 """
 from __future__ import annotations
 
@@ -126,6 +132,84 @@ def plates(n_particles: int = 262_144, nx: int = 51, ny: int = 51, nz: int = 13,
     return Plates("plates", mesh, [cu], g, dt, steps, seed, pos, vel, np.zeros(n_particles, dtype=np.int32),
                   meshgen.linear_potential(mesh, 0.0, 1.0), meshgen.linear_potential(mesh, 1.0, 0.0),
                   bottom_all[on_plate])
+
+
+def thompson_energy(n, rng, u_b, e_max):
+    """Thompson distribution f(E) ~ E / (E + U_b)^3 on [0, e_max] (eV) by
+    inverse CDF: F(E) = E^2 / (E + U_b)^2, E = U_b sqrt(F) / (1 - sqrt(F))."""
+    f_max = (e_max / (e_max + u_b)) ** 2
+    r = np.sqrt(rng.uniform(0.0, f_max, n))
+    return u_b * r / (1.0 - r)
+
+
+@dataclass
+class ChamberWorkload:
+    """Sputtering chamber: target disk on top, substrate plate inside, mixed
+    Ti neutrals from the target and Ar+ in the volume, the field refreshed
+    from a node potential every `refresh` steps."""
+    name: str
+    chamber: meshgen.Chamber
+    species: list
+    gas: Gas
+    dt: float
+    steps: int
+    refresh: int
+    seed: int
+    pos: np.ndarray
+    vel: np.ndarray
+    sp_idx: np.ndarray
+    shape: np.ndarray | None = None  # spatial profile of the potential (built on first use)
+
+    @property
+    def mesh(self):
+        return self.chamber.mesh
+
+    def potential(self, step: int) -> np.ndarray:
+        """Node potential in force from `step` on: the target biased around
+        -300 V (modulated by 20% over 200 steps), decaying over 2 cm below
+        it and to 0 V at the wall.  Refreshed every `refresh` steps."""
+        if self.shape is None:
+            nd = self.chamber.mesh.nodes
+            r2 = (nd[:, 0] ** 2 + nd[:, 1] ** 2) / self.chamber.radius ** 2
+            # wall nodes (r = radius up to the rounding of the map) at exactly 0 V
+            self.shape = np.exp(-(self.chamber.height - nd[:, 2]) / 0.02) * np.where(r2 >= 1.0 - 1e-9, 0.0, 1.0 - r2)
+        k = (step // self.refresh) * self.refresh
+        return (-300.0 * (1.0 + 0.2 * np.sin(2.0 * np.pi * k / 200.0))) * self.shape
+
+
+def chamber(n_particles: int = 16_000_000, n: int = 80, nz: int = 78, ti_fraction: float = 0.5,
+            steps: int = 1000, dt: float = 5e-8, refresh: int = 10, seed: int = 1005) -> ChamberWorkload:
+    """Production-like chamber (80 x 80 x 78 grid: 2,995,200 tets, 518,319
+    nodes, 10,080 substrate triangles).  Ti neutrals (Thompson energies with
+    U_b = 4.89 eV up to 100 eV, cosine law) leave the target disk (r <= 0.05 m)
+    0.1 mm below it; Ar+ (+1e, 1 eV Maxwellian) fill the volume uniformly.  VSS
+    collisions with Ar at 0.3 Pa, 300 K."""
+    ch = meshgen.cylinder_chamber(n=n, nz=nz)
+    rng = np.random.default_rng(seed)
+    n_ti = int(round(n_particles * ti_fraction))
+    n_ar = n_particles - n_ti
+    ti = species("Ti", "Ar", charge_units=0.0)
+    ar = species("Ar", "Ar", charge_units=1.0, diameter=4.17e-10)
+    g = gas("Ar", 0.3, 300.0, "VSS")
+    # Ti from the target disk, cosine law into the chamber
+    rt = 0.05 * np.sqrt(rng.uniform(0.0, 1.0, n_ti))
+    at = rng.uniform(0.0, 2.0 * np.pi, n_ti)
+    pos_ti = np.stack([rt * np.cos(at), rt * np.sin(at), np.full(n_ti, ch.height - 1e-4)], axis=1)
+    e = thompson_energy(n_ti, rng, 4.89, 100.0)
+    speed = np.sqrt(2.0 * e * EV_J / MASS["Ti"])
+    ct = np.sqrt(rng.uniform(0.0, 1.0, n_ti))
+    st = np.sqrt(1.0 - ct * ct)
+    ph = rng.uniform(0.0, 2.0 * np.pi, n_ti)
+    vel_ti = speed[:, None] * np.stack([st * np.cos(ph), st * np.sin(ph), -ct], axis=1)
+    # Ar+ uniform in the volume, Maxwellian
+    ra = 0.999 * ch.radius * np.sqrt(rng.uniform(0.0, 1.0, n_ar))
+    aa = rng.uniform(0.0, 2.0 * np.pi, n_ar)
+    pos_ar = np.stack([ra * np.cos(aa), ra * np.sin(aa), rng.uniform(1e-4, ch.height - 1e-4, n_ar)], axis=1)
+    vel_ar = rng.normal(0.0, np.sqrt(1.0 * EV_J / MASS["Ar"]), size=(n_ar, 3))
+    pos = np.concatenate([pos_ti, pos_ar])
+    vel = np.concatenate([vel_ti, vel_ar])
+    sp = np.concatenate([np.zeros(n_ti, dtype=np.int32), np.ones(n_ar, dtype=np.int32)])
+    return ChamberWorkload("chamber", ch, [ti, ar], g, dt, steps, refresh, seed, pos, vel, sp)
 
 
 @dataclass

@@ -142,6 +142,38 @@ def test_degenerate_starts_on_vertices_edges_faces():
     pr.close()
 
 
+@pytest.mark.parametrize("precision,K", [("fp64", 1), ("fp64", 10), ("mixed", 10)])
+def test_long_steps_across_many_tets(precision, K):
+    """Steps from a hundredth of a tet to five tets long (1e3..5e5 m/s at
+    1e-7 s in 1 cm tets): long segments end far outside the owner, where the
+    barycentric fast path must hand over to the exact walk, many of them
+    leaving through the surface."""
+    w = workloads.small_box(n_particles=10)
+    rng = np.random.default_rng(31)
+    n = 3000
+    pos = rng.uniform(0.002, 0.098, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    vel = d * np.exp(rng.uniform(np.log(1e3), np.log(5e5), n))[:, None]
+    w.pos, w.vel, w.sp_idx = pos, vel, np.zeros(n, dtype=np.int32)
+    pr = Pair(w, steps_per_launch=K, precision=precision)
+    for k in range(2):
+        pr.advance(10)
+        if precision == "fp64":
+            r = pr.compare(f"long steps {precision} K={K} step {pr.step}")
+            assert r["settled"] > 1000
+        else:
+            # FP32 state: the same bar as the mixed-precision test below for
+            # owner/status/hit agreement (fewer than 0.5% of particles differ)
+            g = pr.eng.particles()
+            st = pr.st
+            same = (g["tet"] == st.tet) & (g["status"] == st.status) & (g["hit"] == st.hit)
+            log(f"long steps mixed K={K} step {pr.step}: differ {1.0 - same.mean():.2e}, "
+                f"settled {int((st.status == 1).sum())}")
+            assert 1.0 - same.mean() < 5e-3
+    pr.close()
+
+
 @pytest.mark.parametrize("K", [4, 25])
 def test_fused_launch_parity(K):
     """K steps fused in one launch (state in registers, stragglers finished by

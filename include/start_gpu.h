@@ -161,13 +161,17 @@ typedef struct start_launch_cfg {
     int32_t sort_interval;    /* steps between sorts by tet; 0 = never                        */
     int32_t graph;            /* 1: the launches between two sorts are captured once in a
                                  hipGraph and replayed                                        */
-    int32_t pad;
+    int32_t walk;             /* 1 (default): the fast path also certifies segments
+                                 through several tets, so only edge/vertex passages go
+                                 to the exact straggler kernel; implies block 256.
+                                 0: single-crossing fast path.  Results are identical   */
 } start_launch_cfg;
 int start_set_launch(start_ctx* ctx, const start_launch_cfg* cfg);
 int start_get_launch(start_ctx* ctx, start_launch_cfg* cfg);
 /* Times each candidate configuration (steps per launch, block, persistent,
- * sort interval) for trial_steps steps from a copy of the current state,
- * restores the state, and sets the fastest.  best and ms_per_step may be NULL. */
+ * sort interval, then the multi-tet walk on and off for the best of those)
+ * for trial_steps steps from a copy of the current state, restores the
+ * state, and sets the fastest.  best and ms_per_step may be NULL. */
 int start_autotune(start_ctx* ctx, int32_t trial_steps, start_launch_cfg* best, double* ms_per_step);
 
 /* ---- adjoint of the collisionless push and the surface-hit map ----

@@ -68,7 +68,7 @@ class RunDesc(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [("steps_per_launch", C.c_int32), ("block", C.c_int32), ("persistent", C.c_int32),
-                ("sort_interval", C.c_int32), ("graph", C.c_int32), ("pad", C.c_int32)]
+                ("sort_interval", C.c_int32), ("graph", C.c_int32), ("walk", C.c_int32)]
 
 
 _LIB = None

@@ -108,6 +108,7 @@ struct StepLaunch {
     int persistent;  // 1: `grid` workgroups pull tiles from a queue
     int grid;
     int batch;       // 1: particles of several runs (per-lane run parameters)
+    int walk;        // 1: certified multi-tet walk on the fast path (block 256 only)
 };
 
 struct StepParams {

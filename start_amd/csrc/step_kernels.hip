@@ -611,6 +611,83 @@ __device__ inline int classify(const double lam[4], double tau, int& fneg)
     return kUncertain;
 }
 
+// Certified walk of the segment p0 -> p1 (absolute FP64, p0 strictly inside
+// t) through as many tets as it crosses, with the FP64 barycentric maps, for
+// steps longer than a tet.  In each tet the exit face is the face f with
+// lambda_f(p1) < 0 whose plane the segment meets first: s_f = l0_f / (l0_f -
+// l1_f) is smallest, i.e. D_fg = l0_f l1_g - l0_g l1_f > 0 for every other
+// such g.  Signs and orderings are trusted only beyond their rounding bound:
+// lambda at a point with sum|lambda| = S errs by at most tau * max(1, S/2)
+// (tau covers S <= 2 and the bound is linear in |x - v0| <= diam * S).  The
+// entry point lies inside a face (all other lambdas > 0), so the exit point
+// found this way lies inside the exit face.  Anything closer than the bounds
+// (edges, vertices, end points near a face) is left to the exact walk.
+enum { kWalkInside = 0, kWalkSurface = 1, kWalkOpen = 2, kWalkUnsure = 3 };
+constexpr int kWalkHops = 16;
+
+__device__ inline double walk_tol(double tau, const double l[4])
+{
+    const double s = (fabs(l[0]) + fabs(l[1])) + (fabs(l[2]) + fabs(l[3]));
+    return tau * fmax(1.0, 0.5 * s) * (1.0 + 1e-6);
+}
+
+__device__ inline int walk64(const DevMesh& M, int32_t t, const double* p0, const double* p1, int32_t& out)
+{
+    constexpr double kE = 1.1102230246251565e-16;  // 2^-53
+    const double tau = M.tau;
+    int32_t cur = t;
+    for (int hop = 0; hop < kWalkHops; ++hop) {
+        double l1[4];
+        bary_at(M, cur, p1[0], p1[1], p1[2], l1);
+        const double t1 = walk_tol(tau, l1);
+        int nneg = 0, npos = 0, f1 = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            npos += l1[q] > t1;
+            if (l1[q] < -t1) {
+                ++nneg;
+                f1 = q;
+            }
+        }
+        if (npos + nneg < 4) return kWalkUnsure;  // p1 too close to a face plane of cur
+        if (nneg == 0) {
+            out = cur;
+            return kWalkInside;
+        }
+        int exit_f = f1;
+        if (nneg > 1) {
+            double l0[4];
+            bary_at(M, cur, p0[0], p0[1], p0[2], l0);
+            const double t0 = walk_tol(tau, l0);
+            exit_f = -1;
+            for (int f = 0; f < 4 && exit_f < 0; ++f) {
+                if (!(l1[f] < -t1)) continue;
+                bool first = true;
+                for (int g = 0; g < 4; ++g) {
+                    if (g == f || !(l1[g] < -t1)) continue;
+                    const double a = l0[f] * l1[g], b = l0[g] * l1[f];
+                    const double err = t0 * (fabs(l1[f]) + fabs(l1[g]) + 2.0 * t1) +
+                                       t1 * (fabs(l0[f]) + fabs(l0[g]) + 2.0 * t0) + 4.0 * kE * (fabs(a) + fabs(b));
+                    if (!(a - b > err)) {
+                        first = false;
+                        break;
+                    }
+                }
+                if (first) exit_f = f;
+            }
+            if (exit_f < 0) return kWalkUnsure;
+        }
+        const int32_t code = nth(M.tnbr[cur], exit_f);
+        if (code <= -2) {
+            out = -2 - code;
+            return kWalkSurface;
+        }
+        if (code == -1) return kWalkOpen;
+        cur = code;
+    }
+    return kWalkUnsure;
+}
+
 enum StepOutcome { kDone = 0, kSettledNow = 1, kDeferNow = 2, kFrozenNow = 3 };
 
 // settle record for the adjoint (rare; out of line, scalar arguments)
@@ -643,7 +720,7 @@ struct StepResult {
     int32_t hit_tri;
 };
 
-template <typename R, bool kBatch, typename Sink>
+template <typename R, bool kBatch, bool kWalk, typename Sink>
 __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParams& S, const SpeciesK& k,
                                                const RunK* __restrict__ rk, uint32_t vid, uint32_t step,
                                                int remaining, PState<R>& s, Sink& sink)
@@ -749,6 +826,50 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
             }
         }
     }
+    if (kWalk && !certified && s.t >= 0 && (s.f & kStrict)) {
+        // a segment through several tets: the certified FP64 walk
+        double p0[3], p1[3];
+        abs_pos(M, s.t, s.x, s.y, s.z, p0);
+        abs_pos(M, s.t, x1, y1, z1, p1);
+        int32_t where = -1;
+        const int w = walk64(M, s.t, p0, p1, where);
+        if (w == kWalkSurface) {
+            s.x = x1;  // crosses this surface triangle first
+            s.y = y1;
+            s.z = z1;
+            return StepResult{kSettledNow, where};
+        }
+        if (w != kWalkUnsure) {
+            R nx, ny, nz;
+            to_frame(M, w == kWalkOpen ? -1 : where, p1, nx, ny, nz);
+            if (w == kWalkOpen) {
+                s.t = -1;
+                s.f &= ~kStrict;
+                s.x = nx;
+                s.y = ny;
+                s.z = nz;
+                return StepResult{kDone, -1};
+            }
+            // the position as stored (FP32 in its owner's frame in mixed mode)
+            // must itself be strictly inside the owner the walk found
+            bary_at(M, where, nx, ny, nz, lam);
+            int f2;
+            int c2 = classify(lam, tau, f2);
+            if (!std::is_same<R, double>::value && c2 == kUncertain) {
+                double q[3];
+                abs_pos(M, where, nx, ny, nz, q);
+                bary_at(M, where, q[0], q[1], q[2], lam);
+                c2 = classify(lam, M.tau, f2);
+            }
+            if (c2 == kInside) {
+                s.t = where;
+                x1 = nx;
+                y1 = ny;
+                z1 = nz;
+                certified = true;
+            }
+        }
+    }
     if (!certified) {
         s.x = x1;
         s.y = y1;
@@ -779,7 +900,7 @@ __global__ void prologue_kernel(double* __restrict__ a, double* __restrict__ b, 
 // S.nsteps steps with the state and the owning tet's data in registers.  A
 // particle whose relocation cannot be certified stops and goes to the work
 // list with the step it stopped at.  Called with a block-uniform tile.
-template <typename R, int BS, int HashBits, bool kBatch>
+template <typename R, int BS, int HashBits, bool kBatch, bool kWalk>
 __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R>& P, const StepParams& S,
                                           const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
                                           WorkItem* __restrict__ wl, unsigned long long* __restrict__ cnt,
@@ -823,7 +944,7 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
             for (int ks = 0; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                StepResult r = step_one<R, kBatch>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
+                StepResult r = step_one<R, kBatch, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (r.outcome == kFrozenNow) break;
                 ++processed;
                 moved = true;
@@ -903,7 +1024,7 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
 // a grid sized to the chip pulls tiles from a queue counter until the live
 // prefix is covered (one launch for every run of a batch, whatever their
 // sizes).  The LDS hash has 8 slots per thread (12 B each: 24 KB at BS = 256).
-template <typename R, int BS, bool kBatch, bool kPersistent>
+template <typename R, int BS, bool kBatch, bool kPersistent, bool kWalk>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) step_main(DevMesh M, DevParticles<R> P, StepParams S,
                                                        const SpeciesK* __restrict__ sp,
                                                        double* __restrict__ node_charge, WorkItem* __restrict__ wl,
@@ -932,7 +1053,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) st
     const int64_t n_active = *S.n_active;
     if (!kPersistent) {
         if ((int64_t)blockIdx.x * BS < n_active)  // else the whole block is past the live particles
-            step_tile<R, BS, kBits, kBatch>(M, P, S, sp, node_charge, wl, cnt, tri_count, blockIdx.x, n_active, step0,
+            step_tile<R, BS, kBits, kBatch, kWalk>(M, P, S, sp, node_charge, wl, cnt, tri_count, blockIdx.x, n_active, step0,
                                             &blk_processed, hk, hv);
         return;
     }
@@ -941,7 +1062,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) st
         __syncthreads();
         const int64_t tile = s_tile;
         if (tile * BS >= n_active) break;  // queue drained: every workgroup exits here
-        step_tile<R, BS, kBits, kBatch>(M, P, S, sp, node_charge, wl, cnt, tri_count, tile, n_active, step0,
+        step_tile<R, BS, kBits, kBatch, kWalk>(M, P, S, sp, node_charge, wl, cnt, tri_count, tile, n_active, step0,
                                         &blk_processed, hk, hv);
     }
 }
@@ -982,8 +1103,9 @@ __device__ inline bool finish_exact(const DevMesh& M, const StepParams& S, const
 
 // Straggler kernel: finishes each deferred relocation exactly, then runs the
 // particle's remaining steps of the launch with the certified fast path,
-// resolving exactly again whenever a step defers.
-template <typename R>
+// resolving exactly again whenever a step defers.  (Holding it to 128 VGPRs
+// for 4 waves per SIMD spills ~300 registers and was slower than 1 wave.)
+template <typename R, bool kWalk>
 __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P, StepParams S,
                                                    const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
                                                    const WorkItem* __restrict__ wl,
@@ -1021,7 +1143,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
             for (int ks = it.k + 1; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                StepResult rr = step_one<R, true>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
+                StepResult rr = step_one<R, true, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (rr.outcome == kFrozenNow) break;
                 ++extra;
                 if (rr.outcome == kDeferNow || rr.outcome == kSettledNow) {
@@ -1360,26 +1482,36 @@ void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, cons
 {
     if (P.n > 0) {
         const unsigned grid = L.persistent ? (unsigned)L.grid : blocks_for(P.n, L.block == 128 ? 128 : 256);
-        const int v = (L.block == 128 ? 4 : 0) + (L.batch ? 2 : 0) + (L.persistent ? 1 : 0);
+        // the multi-tet walk is instantiated for 256-thread workgroups only
+        // (the host sets block = 256 with it)
+        const int v = L.walk ? 8 + (L.batch ? 2 : 0) + (L.persistent ? 1 : 0)
+                             : (L.block == 128 ? 4 : 0) + (L.batch ? 2 : 0) + (L.persistent ? 1 : 0);
         switch (v) {
-#define SG_STEP_MAIN(BS_, B_, P_) \
-    step_main<R, BS_, B_, P_><<<grid, BS_, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count)
-        case 0: SG_STEP_MAIN(256, false, false); break;
-        case 1: SG_STEP_MAIN(256, false, true); break;
-        case 2: SG_STEP_MAIN(256, true, false); break;
-        case 3: SG_STEP_MAIN(256, true, true); break;
-        case 4: SG_STEP_MAIN(128, false, false); break;
-        case 5: SG_STEP_MAIN(128, false, true); break;
-        case 6: SG_STEP_MAIN(128, true, false); break;
-        default: SG_STEP_MAIN(128, true, true); break;
+#define SG_STEP_MAIN(BS_, B_, P_, W_) \
+    step_main<R, BS_, B_, P_, W_><<<grid, BS_, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count)
+        case 0: SG_STEP_MAIN(256, false, false, false); break;
+        case 1: SG_STEP_MAIN(256, false, true, false); break;
+        case 2: SG_STEP_MAIN(256, true, false, false); break;
+        case 3: SG_STEP_MAIN(256, true, true, false); break;
+        case 4: SG_STEP_MAIN(128, false, false, false); break;
+        case 5: SG_STEP_MAIN(128, false, true, false); break;
+        case 6: SG_STEP_MAIN(128, true, false, false); break;
+        case 7: SG_STEP_MAIN(128, true, true, false); break;
+        case 8: SG_STEP_MAIN(256, false, false, true); break;
+        case 9: SG_STEP_MAIN(256, false, true, true); break;
+        case 10: SG_STEP_MAIN(256, true, false, true); break;
+        default: SG_STEP_MAIN(256, true, true, true); break;
 #undef SG_STEP_MAIN
         }
     }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
-    // the work list is only known on the device: a grid that covers up to
-    // 65,536 deferred particles at once, the rest by grid stride
-    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 1024u));
-    step_resolve<R><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
+    // the work list is only known on the device: a grid of up to 4096
+    // single-wave workgroups, the rest of the list by grid stride
+    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 4096u));
+    if (L.walk)
+        step_resolve<R, true><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
+    else
+        step_resolve<R, false><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
 }
 
 void launch_locate(hipStream_t s, const DevMesh& M, const double* x, const double* y, const double* z, int64_t n,

@@ -134,14 +134,20 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_read_tri_counts_runs(self.ctx, _up(out)))
         return out[:, : self.mesh.n_tris]
 
-    def set_launch(self, steps_per_launch=1, block=256, persistent=False, sort_interval=20, graph=False):
-        cfg = _lib.LaunchCfg(int(steps_per_launch), int(block), int(persistent), int(sort_interval), int(graph), 0)
+    def set_launch(self, steps_per_launch=1, block=256, persistent=False, sort_interval=20, graph=False,
+                   walk=None):
+        """Launch shape; walk=None keeps the multi-tet walk setting (on unless
+        switched off)."""
+        if walk is None:
+            walk = self.get_launch()["walk"]
+        cfg = _lib.LaunchCfg(int(steps_per_launch), int(block), int(persistent), int(sort_interval), int(graph),
+                             int(walk))
         self._check(_lib.lib().start_set_launch(self.ctx, C.byref(cfg)))
 
     def get_launch(self) -> dict:
         cfg = _lib.LaunchCfg()
         self._check(_lib.lib().start_get_launch(self.ctx, C.byref(cfg)))
-        return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_ if k != "pad"}
+        return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_}
 
     def autotune(self, trial_steps: int = 200):
         """Project autotuner: times the launch configurations from the current
@@ -150,7 +156,7 @@ class ParticleInCellStep:
         cfg = _lib.LaunchCfg()
         ms = C.c_double(0)
         self._check(_lib.lib().start_autotune(self.ctx, int(trial_steps), C.byref(cfg), C.byref(ms)))
-        return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_ if k != "pad"}, ms.value
+        return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_}, ms.value
 
     def record_settle(self, on: bool = True):
         self._check(_lib.lib().start_record_settle(self.ctx, int(on)))

@@ -22,7 +22,7 @@ class BatchPair:
     """A batch on the GPU and one oracle simulation per run."""
 
     def __init__(self, sizes, scales, concs, keys, K=1, sort_interval=10, block=256, persistent=False,
-                 graph=False, precision="fp64", model="VSS", seed=1004):
+                 graph=False, precision="fp64", model="VSS", seed=1004, walk=None):
         w = workloads.large_box(n_particles=sum(sizes), cells=10, model=model, seed=seed)
         w.dt = 2e-8
         self.w, self.sizes, self.K = w, sizes, K
@@ -52,6 +52,9 @@ class BatchPair:
         self.eng.set_field_basis(0, E=E0)
         self.eng.set_field_basis(1, E=E1)
         self.eng.load_batch(runs, w.pos, w.vel, w.sp_idx)
+        if walk is not None:  # the multi-tet walk on (the default) or off
+            self.eng.set_launch(steps_per_launch=K, block=block, persistent=persistent, sort_interval=sort_interval,
+                                graph=graph, walk=walk)
         self.step = 0
 
     def advance(self, k):
@@ -100,15 +103,17 @@ CONCS = [2.4e22, 2.4e20, 1.2e23, 6.0e21]
 KEYS = [0, 1, 7, 1004]
 
 
-@pytest.mark.parametrize("K,block,persistent,graph", [(1, 256, False, False), (5, 128, True, True),
-                                                      (10, 256, True, False), (4, 128, False, True)])
-def test_batch_runs_match_independent_oracle_runs(K, block, persistent, graph):
+@pytest.mark.parametrize("K,block,persistent,graph,walk", [(1, 256, False, False, 1), (5, 128, True, True, 0),
+                                                           (10, 256, True, False, 1), (4, 128, False, True, 0),
+                                                           (10, 256, True, True, 0), (4, 256, False, False, 0)])
+def test_batch_runs_match_independent_oracle_runs(K, block, persistent, graph, walk):
     bp = BatchPair(SIZES, SCALES, CONCS, KEYS, K=K, sort_interval=2 * K, block=block, persistent=persistent,
-                   graph=graph)
+                   graph=graph, walk=walk)
     settled = 0
     for _ in range(3):
         bp.advance(20)
-        settled = bp.compare(f"batch K={K} block={block} persistent={persistent} graph={graph} step {bp.step}")
+        settled = bp.compare(f"batch K={K} block={block} persistent={persistent} graph={graph} walk={walk} "
+                             f"step {bp.step}")
     assert settled > 20
     bp.close()
 

@@ -142,12 +142,13 @@ def test_degenerate_starts_on_vertices_edges_faces():
     pr.close()
 
 
-@pytest.mark.parametrize("precision,K", [("fp64", 1), ("fp64", 10), ("mixed", 10)])
-def test_long_steps_across_many_tets(precision, K):
+@pytest.mark.parametrize("precision,K,walk", [("fp64", 1, 1), ("fp64", 10, 1), ("fp64", 10, 0), ("mixed", 10, 1)])
+def test_long_steps_across_many_tets(precision, K, walk):
     """Steps from a hundredth of a tet to five tets long (1e3..5e5 m/s at
-    1e-7 s in 1 cm tets): long segments end far outside the owner, where the
-    barycentric fast path must hand over to the exact walk, many of them
-    leaving through the surface."""
+    1e-7 s in 1 cm tets): long segments end far outside the owner.  With the
+    multi-tet walk (the default) the fast path certifies most of them,
+    without it they go to the exact walk; many leave through the surface.
+    Results are identical either way."""
     w = workloads.small_box(n_particles=10)
     rng = np.random.default_rng(31)
     n = 3000
@@ -157,11 +158,16 @@ def test_long_steps_across_many_tets(precision, K):
     vel = d * np.exp(rng.uniform(np.log(1e3), np.log(5e5), n))[:, None]
     w.pos, w.vel, w.sp_idx = pos, vel, np.zeros(n, dtype=np.int32)
     pr = Pair(w, steps_per_launch=K, precision=precision)
+    assert pr.eng.get_launch()["walk"] == 1  # the default
+    pr.eng.set_launch(steps_per_launch=K, block=256, persistent=False, sort_interval=20, walk=walk)
     for k in range(2):
         pr.advance(10)
         if precision == "fp64":
-            r = pr.compare(f"long steps {precision} K={K} step {pr.step}")
+            r = pr.compare(f"long steps {precision} K={K} walk={walk} step {pr.step}")
             assert r["settled"] > 1000
+            # the walk certifies most multi-tet segments
+            frac = r["deferred"] / r["processed"]
+            assert (frac < 0.02) if walk else (frac > 0.05), frac
         else:
             # FP32 state: the same bar as the mixed-precision test below for
             # owner/status/hit agreement (fewer than 0.5% of particles differ)
@@ -174,14 +180,16 @@ def test_long_steps_across_many_tets(precision, K):
     pr.close()
 
 
-@pytest.mark.parametrize("K", [4, 25])
-def test_fused_launch_parity(K):
+@pytest.mark.parametrize("K,walk", [(4, 0), (25, 0), (25, 1)])
+def test_fused_launch_parity(K, walk):
     """K steps fused in one launch (state in registers, stragglers finished by
-    the exact kernel): same bit-exact bar, charge summed over each launch."""
+    the exact kernel): same bit-exact bar, charge summed over each launch;
+    with the single-crossing fast path and with the multi-tet walk."""
     w = workloads.large_box(n_particles=20_000, cells=10, model="VSS")
     w.gas.concentration *= 100.0
     w.dt = 2e-8
     pr = Pair(w, steps_per_launch=K)
+    pr.eng.set_launch(steps_per_launch=K, block=256, persistent=False, sort_interval=20, walk=walk)
     for k in range(2):
         pr.advance(100)
         r = pr.compare(f"fused K={K} step {pr.step}")

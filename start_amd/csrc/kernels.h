@@ -78,6 +78,7 @@ struct DevMesh {
     const int32_t* sgrid_start;
     const int32_t* sgrid_list;
     GridInfo grid;
+    double node_lo[3], node_hi[3];  // bounding box of the nodes (every tet and triangle is inside)
     int64_t n_nodes, n_tets, n_tris;
     double tau;
     float tau32;

@@ -50,6 +50,7 @@ struct HostMesh {
     std::vector<double> geo;      // 12T: v0 (3), A row-major (9)
     std::vector<uint8_t> tri_degenerate;
     double tau = 0.0;             // certification threshold of the map
+    double node_lo[3]{}, node_hi[3]{};  // bounding box of the nodes
     GridInfo grid{};
     std::vector<int32_t> tgrid_start, tgrid_list;
     std::vector<int32_t> sgrid_start, sgrid_list;

@@ -217,6 +217,8 @@ std::string prepare_mesh(HostMesh& m, int64_t n_nodes, const double* nodes, int6
     double vol = std::max((mx[0] - mn[0]) * (mx[1] - mn[1]) * (mx[2] - mn[2]), 1e-300);
     double edge = cbrt(vol / (double)n_tets * 6.0);
     for (int k = 0; k < 3; ++k) {
+        m.node_lo[k] = mn[k];
+        m.node_hi[k] = mx[k];
         m.grid.lo[k] = mn[k];
         int d = (int)ceil((mx[k] - mn[k]) / edge);
         m.grid.div[k] = std::max(1, std::min(d, 1024));

@@ -175,6 +175,10 @@ struct start_ctx {
         M.sgrid_start = d_sgs;
         M.sgrid_list = d_sgl;
         M.grid = hm.grid;
+        for (int k = 0; k < 3; ++k) {
+            M.node_lo[k] = hm.node_lo[k];
+            M.node_hi[k] = hm.node_hi[k];
+        }
         M.n_nodes = hm.n_nodes;
         M.n_tets = hm.n_tets;
         M.n_tris = hm.n_tris;

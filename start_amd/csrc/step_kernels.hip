@@ -826,6 +826,24 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
             }
         }
     }
+    if (s.t < 0) {
+        // outside the mesh: a segment wholly beyond one face of the nodes'
+        // bounding box meets no tet and no surface triangle (exact compares)
+        double p0[3], p1[3];
+        abs_pos(M, -1, s.x, s.y, s.z, p0);
+        abs_pos(M, -1, x1, y1, z1, p1);
+        bool away = false;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            away = away || (p0[q] < M.node_lo[q] && p1[q] < M.node_lo[q]) ||
+                   (p0[q] > M.node_hi[q] && p1[q] > M.node_hi[q]);
+        if (away) {
+            s.x = x1;
+            s.y = y1;
+            s.z = z1;
+            return StepResult{kDone, -1};
+        }
+    }
     if (kWalk && !certified && s.t >= 0 && (s.f & kStrict)) {
         // a segment through several tets: the certified FP64 walk
         double p0[3], p1[3];

@@ -180,6 +180,40 @@ def test_long_steps_across_many_tets(precision, K, walk):
     pr.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_open_sides_particles_leave_and_fly_on(precision):
+    """Surface triangles on the top and bottom only: particles leave through
+    the open sides and keep moving outside the mesh (no tet, no field, no
+    charge) until they leave the expanded surface bounding box; the ones
+    whose segments stay beyond a face of the mesh's bounding box take the
+    fast exit path, the others the exact search."""
+    w = workloads.small_box(n_particles=10, charge_units=1.0)
+    m = w.mesh
+    z = m.nodes[m.tris][:, :, 2]
+    keep = np.all(z == 0.0, axis=1) | np.all(z == z.max(), axis=1)
+    w.mesh = meshgen.TetMesh(m.nodes, m.tets, np.ascontiguousarray(m.tris[keep]))
+    rng = np.random.default_rng(44)
+    n = 4000
+    pos = rng.uniform(0.003, 0.097, size=(n, 3))
+    vel = rng.normal(0.0, 1.0, size=(n, 3)) * np.array([2e4, 2e4, 3e3])
+    w.pos, w.vel, w.sp_idx = pos, vel, np.zeros(n, dtype=np.int32)
+    pr = Pair(w, precision=precision, steps_per_launch=10)
+    for k in range(3):
+        pr.advance(10)
+        if precision == "fp64":
+            r = pr.compare(f"open sides fp64 step {pr.step}")
+        else:
+            g = pr.eng.particles()
+            st = pr.st
+            same = (g["tet"] == st.tet) & (g["status"] == st.status) & (g["hit"] == st.hit)
+            log(f"open sides mixed step {pr.step}: differ {1.0 - same.mean():.2e}")
+            assert 1.0 - same.mean() < 5e-3
+    outside = int(((pr.st.tet < 0) & (pr.st.status == 0)).sum())
+    log(f"open sides {precision}: {outside} particles outside the mesh, settled {int((pr.st.status == 1).sum())}")
+    assert outside > 500
+    pr.close()
+
+
 @pytest.mark.parametrize("K,walk", [(4, 0), (25, 0), (25, 1)])
 def test_fused_launch_parity(K, walk):
     """K steps fused in one launch (state in registers, stragglers finished by

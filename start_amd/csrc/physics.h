@@ -95,6 +95,9 @@ template <>
 __device__ inline double m_sqrt<double>(double x) { return sqrt(x); }
 template <>
 __device__ inline float m_sqrt<float>(float x) { return sqrtf(x); }
+// next representable value from x towards y (x itself when y == x)
+__device__ inline double m_nextafter(double x, double y) { return nextafter(x, y); }
+__device__ inline float m_nextafter(float x, float y) { return nextafterf(x, y); }
 template <typename R>
 __device__ inline R m_pow(R x, R y);
 template <>

@@ -306,12 +306,26 @@ __device__ inline void bary64(const double* __restrict__ g, double x, double y, 
     lam[0] = 1.0 - ((lam[1] + lam[2]) + lam[3]);
 }
 
+// FP64 atomic adds with the address space stated: a pointer that reaches the
+// add through a struct is generic to the compiler, which then emits a FLAT
+// atomic (slower, and for LDS it waits on both memory counters).
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(1))) double global_f64;
+__device__ inline void lds_add_f64(double* p, double v)
+{
+    __hip_atomic_fetch_add((lds_f64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);  // ds_add_f64
+}
+__device__ inline void global_add_f64(double* p, double v)
+{
+    __hip_atomic_fetch_add((global_f64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // global_atomic_add_f64
+}
+
 __device__ inline void scatter4(double* __restrict__ node_charge, const int4& nv, double q, const double lam[4])
 {
-    unsafeAtomicAdd(&node_charge[nv.x], q * lam[0]);
-    unsafeAtomicAdd(&node_charge[nv.y], q * lam[1]);
-    unsafeAtomicAdd(&node_charge[nv.z], q * lam[2]);
-    unsafeAtomicAdd(&node_charge[nv.w], q * lam[3]);
+    global_add_f64(&node_charge[nv.x], q * lam[0]);
+    global_add_f64(&node_charge[nv.y], q * lam[1]);
+    global_add_f64(&node_charge[nv.z], q * lam[2]);
+    global_add_f64(&node_charge[nv.w], q * lam[3]);
 }
 
 // Block-local reduction of the charge scatter: contributions to the same node
@@ -324,15 +338,19 @@ __device__ inline void lds_charge_add(int32_t* hk, double* hv, double* __restric
                                       double val)
 {
     uint32_t h = ((uint32_t)node * 2654435761u) >> (32 - Bits);
+    // (left to the compiler to unroll: a rolled loop made the FP64 streaming
+    // kernel 7% slower)
+    // (reading the slot before the compare-and-swap, to skip it for nodes
+    // already present, made the kernel 17-35% slower)
     for (int probe = 0; probe < 32; ++probe) {
         int32_t prev = atomicCAS(&hk[h], -1, node);
         if (prev == -1 || prev == node) {
-            atomicAdd(&hv[h], val);
+            lds_add_f64(&hv[h], val);
             return;
         }
         h = (h + 1) & ((1u << Bits) - 1u);
     }
-    unsafeAtomicAdd(&node_charge[node], val);  // crowded table: straight to memory
+    global_add_f64(&node_charge[node], val);  // crowded table: straight to memory
 }
 
 // base: offset of the run's node-charge array (0 for a single simulation)
@@ -850,7 +868,20 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
         abs_pos(M, s.t, s.x, s.y, s.z, p0);
         abs_pos(M, s.t, x1, y1, z1, p1);
         int32_t where = -1;
-        const int w = walk64(M, s.t, p0, p1, where);
+        int w = walk64(M, s.t, p0, p1, where);
+        if (!std::is_same<R, double>::value && w == kWalkUnsure) {
+            // FP32 end points lie on a lattice (~1e-9 m apart in a 1 cm tet),
+            // and about 1e-7 of them land exactly on a face plane (one through
+            // v0, e.g. local x == y).  Such a point is moved one FP32 ulp
+            // further along the step (within the FP32 state's own rounding)
+            // and certified again; a point that stays uncertain goes to the
+            // exact walk as before.
+            x1 = m_nextafter(x1, x1 + s.vx);
+            y1 = m_nextafter(y1, y1 + s.vy);
+            z1 = m_nextafter(z1, z1 + s.vz);
+            abs_pos(M, s.t, x1, y1, z1, p1);
+            w = walk64(M, s.t, p0, p1, where);
+        }
         if (w == kWalkSurface) {
             s.x = x1;  // crosses this surface triangle first
             s.y = y1;
@@ -878,6 +909,27 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
                 abs_pos(M, where, nx, ny, nz, q);
                 bary_at(M, where, q[0], q[1], q[2], lam);
                 c2 = classify(lam, M.tau, f2);
+            }
+            if (!std::is_same<R, double>::value) {
+                // Rounding the end point to FP32 in the new frame can put the
+                // stored point (not the segment's end) just beyond a face of
+                // `where`; it is then owned by the neighbour across that face,
+                // where it is stored rounded in that neighbour's frame.
+                for (int tries = 0; tries < 2 && c2 == kOneFace; ++tries) {
+                    const int32_t code = nth(M.tnbr[where], f2);
+                    if (code < 0) break;  // a surface or open face: the exact walk decides
+                    double q[3];
+                    abs_pos(M, where, nx, ny, nz, q);
+                    where = code;
+                    to_frame(M, where, q, nx, ny, nz);
+                    bary_at(M, where, nx, ny, nz, lam);
+                    c2 = classify(lam, tau, f2);
+                    if (c2 == kUncertain) {
+                        abs_pos(M, where, nx, ny, nz, q);
+                        bary_at(M, where, q[0], q[1], q[2], lam);
+                        c2 = classify(lam, M.tau, f2);
+                    }
+                }
             }
             if (c2 == kInside) {
                 s.t = where;
@@ -1032,7 +1084,7 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
     if (S.charge) {
         for (int h = threadIdx.x; h < kSlots; h += BS) {
             int32_t node = hk[h];
-            if (node >= 0) unsafeAtomicAdd(&node_charge[node], hv[h]);
+            if (node >= 0) global_add_f64(&node_charge[node], hv[h]);
         }
     }
     __syncthreads();  // the LDS table and counter are reused by the next tile
@@ -1264,7 +1316,7 @@ __global__ void tet_occupancy_kernel(DevParticles<R> P, const SpeciesK* __restri
     if ((f & kSettled) || t < 0) return;
     atomicAdd(&tet_n[t], 1);
     double q = sp[(f >> kSpeciesShift) & 0xff].charge;
-    if (q != 0.0) unsafeAtomicAdd(&tet_q[t], q);
+    if (q != 0.0) global_add_f64(&tet_q[t], q);
 }
 
 // rho_n = sum(rho_t V_t) / sum V_t over the occupied tets around n, ascending tet id
@@ -1476,8 +1528,8 @@ __global__ void __launch_bounds__(256) adjoint_push_kernel(DevMesh M, DevParticl
         g1 += __shfl_xor(g1, off);
     }
     if ((threadIdx.x & 63) == 0 && (g0 != 0.0 || g1 != 0.0)) {
-        unsafeAtomicAdd(&grad[0], g0);
-        unsafeAtomicAdd(&grad[1], g1);
+        global_add_f64(&grad[0], g0);
+        global_add_f64(&grad[1], g1);
     }
     if (err) atomicOr(&cnt[kCntError], 1ull);
 }

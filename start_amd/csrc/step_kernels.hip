@@ -872,13 +872,17 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
         if (!std::is_same<R, double>::value && w == kWalkUnsure) {
             // FP32 end points lie on a lattice (~1e-9 m apart in a 1 cm tet),
             // and about 1e-7 of them land exactly on a face plane (one through
-            // v0, e.g. local x == y).  Such a point is moved one FP32 ulp
-            // further along the step (within the FP32 state's own rounding)
-            // and certified again; a point that stays uncertain goes to the
-            // exact walk as before.
-            x1 = m_nextafter(x1, x1 + s.vx);
-            y1 = m_nextafter(y1, y1 + s.vy);
-            z1 = m_nextafter(z1, z1 + s.vz);
+            // v0, e.g. local x == y).  Such a point is moved by 1, 2 and 3 FP32
+            // ulps along the step in x, y and z (unequal, so that it leaves
+            // planes like x == y, which an equal move would keep it on) -- within
+            // the FP32 state's own rounding -- and certified again; a point that
+            // stays uncertain goes to the exact walk as before.
+            const R ux = s.vx < R(0) ? R(-INFINITY) : R(INFINITY);
+            const R uy = s.vy < R(0) ? R(-INFINITY) : R(INFINITY);
+            const R uz = s.vz < R(0) ? R(-INFINITY) : R(INFINITY);
+            x1 = m_nextafter(x1, ux);
+            y1 = m_nextafter(m_nextafter(y1, uy), uy);
+            z1 = m_nextafter(m_nextafter(m_nextafter(z1, uz), uz), uz);
             abs_pos(M, s.t, x1, y1, z1, p1);
             w = walk64(M, s.t, p0, p1, where);
         }

@@ -552,6 +552,33 @@ struct RegSink {
         a2 += q * lam[2];
         a3 += q * lam[3];
     }
+    // The flush at the end of a tile, where every lane of the wavefront is
+    // active: lanes of one run of equal (tet, run offset) -- neighbours in the
+    // tet-sorted order -- sum their four weights into the run's first lane
+    // (segmented Hillis-Steele over the wavefront), which alone goes to the
+    // inner sink.  One LDS insertion per run instead of one per particle.
+    __device__ void flush_wave(const DevMesh& M)
+    {
+        const int lane = threadIdx.x & 63;
+        const int32_t up_t = __shfl_up(cur, 1), up_b = __shfl_up(base, 1);
+        const bool head = lane == 0 || up_t != cur || up_b != base;
+        const unsigned long long hm = __ballot(head);
+        const int seg = __popcll(hm & (~0ull >> (63 - lane)));  // runs starting at or below this lane
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int sd = __shfl_down(seg, d);
+            const double b0 = __shfl_down(a0, d), b1 = __shfl_down(a1, d), b2 = __shfl_down(a2, d),
+                         b3 = __shfl_down(a3, d);
+            if (lane + d < 64 && sd == seg) {
+                a0 += b0;
+                a1 += b1;
+                a2 += b2;
+                a3 += b3;
+            }
+        }
+        if (!head) cur = -1;  // summed into the run's first lane
+        flush(M);
+    }
 };
 
 // Collision decision u1 < P (HSModel.cpp:15-19, VHSModel.cpp:60-70).  For
@@ -1054,7 +1081,7 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
             }
         }
     }
-    if (S.charge) sink.flush(M);
+    if (S.charge) sink.flush_wave(M);  // every lane is active here
     // work list: one atomic per wavefront, lanes take consecutive slots
     unsigned long long dm = __ballot(deferred);
     if (dm) {

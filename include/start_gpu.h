@@ -128,6 +128,60 @@ int start_load_particles(start_ctx* ctx, int64_t n, const double* x, const doubl
                          const double* z, const double* vx, const double* vy, const double* vz,
                          const int32_t* species, const uint64_t* ids);
 
+/* ---- particle sources sampled on the device ----
+ * Replaces ParticleGeneratorHost::fromPointSource / fromSurfaceSource
+ * (ParticleGeneratorHost.cpp:33-159), their CUDA twins
+ * (ParticleGeneratorDevice.cu:56-304) and ParticleDisplacer
+ * (ParticleDisplacer.cpp:48-134): the particles are drawn straight into the
+ * context's arrays (no host round trip) and located, as by
+ * start_load_particles.  Ids are 0..n-1 in source order.
+ *
+ * Energies: START_ENERGY_MONO (every particle at energy_eV, the reference's
+ * sources), START_ENERGY_MAXWELLIAN (velocity components normal with variance
+ * kT/m, kT = p0 eV; the direction fields are not used),
+ * START_ENERGY_THOMPSON (f(E) ~ E/(E+U)^3 on [0, E_max], U = p0, E_max = p1,
+ * eV), START_ENERGY_LOGNORMAL (mean p0 eV, log-standard deviation p1).
+ * Directions about the source direction d = (sin t cos p, sin t sin p, cos t)
+ * (t = theta, p = phi; a surface cell's normal n for surface sources, with
+ * t = acos(n_z/|n|), p = atan2(n_y, n_x) as ParticleGeneratorHost.cpp:139-146):
+ * START_ANGLE_FAN (the reference: polar angle t + u*expansion with u uniform
+ * in [-1, 1), azimuth p; ParticleDynamicUtils.hpp:52-96),
+ * START_ANGLE_COSINE (Lambert law about d), START_ANGLE_ISOTROPIC,
+ * START_ANGLE_CONE (uniform in solid angle within half-angle expansion of d).
+ * displace: each particle starts at its source point + displace * d / max|d_i|
+ * (ParticleDisplacer.cpp:20-31 uses 0.05 m; 0 leaves it on the source).
+ * A surface source spreads count over its cells as the reference does
+ * (count / n_cells each, the remainder to cells picked at random, seeded).
+ * Variates: Philox4x32-10, counter {particle index, 1, source index, draw},
+ * key {seed_lo, seed_hi}: disjoint from the step's variates. */
+#define START_SOURCE_POINT 0
+#define START_SOURCE_SURFACE 1
+#define START_ENERGY_MONO 0
+#define START_ENERGY_MAXWELLIAN 1
+#define START_ENERGY_THOMPSON 2
+#define START_ENERGY_LOGNORMAL 3
+#define START_ANGLE_FAN 0
+#define START_ANGLE_COSINE 1
+#define START_ANGLE_ISOTROPIC 2
+#define START_ANGLE_CONE 3
+typedef struct start_source_desc {
+    int32_t kind;          /* START_SOURCE_*                                        */
+    int32_t species;       /* index into start_params.species                       */
+    int64_t count;         /* particles, > 0 (ParticleGeneratorHost.cpp:55-59)      */
+    int32_t energy;        /* START_ENERGY_*                                        */
+    int32_t angle;         /* START_ANGLE_*                                         */
+    double energy_eV;      /* START_ENERGY_MONO: the energy; != 0 (:61-66)          */
+    double p0, p1;         /* distribution parameters (above)                       */
+    double position[3];    /* point source                                          */
+    double theta, phi;     /* point source direction, radians                       */
+    double expansion;      /* fan / cone half-angle, radians                        */
+    double displace;       /* start offset along d (m); 0 = none                    */
+    const double* centres; /* surface source: 3 * n_cells cell centres              */
+    const double* normals; /* surface source: 3 * n_cells outward normals (any length > 0) */
+    int64_t n_cells;
+} start_source_desc;
+int start_generate(start_ctx* ctx, int32_t n_sources, const start_source_desc* sources, uint64_t seed);
+
 /* ---- batched sweeps: independent runs on one mesh in the same launches ----
  * (no counterpart in the reference, which runs one simulation per process;
  * each run behaves exactly like a context of its own loaded with that run's

@@ -27,8 +27,11 @@ EXPORTS = [
     "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
     "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
-    "start_read_tet_field",
+    "start_read_tet_field", "start_generate",
 ]
+SOURCE = {"point": 0, "surface": 1}
+ENERGY = {"mono": 0, "maxwellian": 1, "thompson": 2, "lognormal": 3}
+ANGLE = {"fan": 0, "cosine": 1, "isotropic": 2, "cone": 3}
 DENSITY = {"barycentric": 0, "tet_average": 1}
 
 
@@ -64,6 +67,15 @@ class Stats(C.Structure):
 class RunDesc(C.Structure):
     _fields_ = [("n_particles", C.c_int64), ("field_scale", C.c_double * 2), ("concentration", C.c_double),
                 ("run", C.c_uint32), ("pad", C.c_int32)]
+
+
+class SourceDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("species", C.c_int32), ("count", C.c_int64),
+                ("energy", C.c_int32), ("angle", C.c_int32), ("energy_eV", C.c_double),
+                ("p0", C.c_double), ("p1", C.c_double), ("position", C.c_double * 3),
+                ("theta", C.c_double), ("phi", C.c_double), ("expansion", C.c_double),
+                ("displace", C.c_double), ("centres", C.POINTER(C.c_double)),
+                ("normals", C.POINTER(C.c_double)), ("n_cells", C.c_int64)]
 
 
 class LaunchCfg(C.Structure):
@@ -112,6 +124,7 @@ def lib() -> C.CDLL:
     L.start_load_batch.argtypes = [vp, i32, P(RunDesc)] + [P(d)] * 6 + [P(i32), P(u64)]
     L.start_set_tet_field_basis.argtypes = [vp, i32, P(d)]
     L.start_read_tet_field.argtypes = [vp, i32, P(d)]
+    L.start_generate.argtypes = [vp, i32, P(SourceDesc), u64]
     L.start_set_node_potential_basis.argtypes = [vp, i32, P(d)]
     L.start_read_node_charge_runs.argtypes = [vp, P(d)]
     L.start_read_tri_counts_runs.argtypes = [vp, P(u64)]

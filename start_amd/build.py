@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libstart_gpu.so")
-SOURCES = ["start_gpu.hip", "step_kernels.hip", "mesh_prep.hip", "sort.hip"]
+SOURCES = ["start_gpu.hip", "step_kernels.hip", "mesh_prep.hip", "sort.hip", "sources.hip"]
 
 
 def hipcc() -> str:

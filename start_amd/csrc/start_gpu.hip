@@ -14,6 +14,7 @@
 
 #include "kernels.h"
 #include "mesh_prep.h"
+#include "sources.h"
 
 namespace sg {
 void launch_zero_charge(hipStream_t s, double* a, double* b, int64_t n);
@@ -646,6 +647,75 @@ int check_cfg(start_ctx* c, const start_launch_cfg& g)
     return START_OK;
 }
 
+// Allocates a population of n particles in runs rk (both buffer sets, work
+// list, sort scratch), clears the counters and outputs and resets the step
+// state; the arrays are filled by the caller.
+int begin_population(start_ctx* c, const std::vector<RunK>& rk, int64_t n, const uint64_t* ids)
+{
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    free_particles(c);
+    c->runs = rk;
+    int rc = upload_runs(c);
+    if (rc != START_OK) return rc;
+    rc = alloc_run_outputs(c);
+    if (rc != START_OK) return rc;
+    hipError_t he = alloc_pbuf(c->A, n, c->es);
+    if (he == hipSuccess) he = dalloc(&c->d_wl, n);
+    if (he == hipSuccess) he = alloc_pbuf(c->B, n, c->es);
+    if (he == hipSuccess) he = dalloc(&c->d_keys, n);
+    if (he == hipSuccess) he = dalloc(&c->d_keys_out, n);
+    if (he == hipSuccess) he = dalloc(&c->d_idx, n);
+    if (he == hipSuccess) he = dalloc(&c->d_perm, n);
+    if (he == hipSuccess) {
+        int bits = 1;
+        while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
+        c->sort_bits = bits;
+        c->sort_temp = sort_temp_bytes(n, bits);
+        he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
+    }
+    if (he != hipSuccess) {
+        free_particles(c);
+        return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("particle allocation failed: ") + hipGetErrorString(he));
+    }
+    c->n = n;
+    c->ids.assign(ids ? ids : (const uint64_t*)nullptr, ids ? ids + n : (const uint64_t*)nullptr);
+    if (!ids) {
+        c->ids.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) c->ids[(size_t)i] = (uint64_t)i;
+    }
+    const int64_t CL = c->charge_len();
+    HIPCHECK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_tri_count, 0,
+                               sizeof(unsigned long long) * c->n_runs() * std::max<int64_t>(c->hm.n_tris, 1),
+                               c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_node_charge, 0, sizeof(double) * 2 * CL, c->stream));
+    if (c->record) {  // no particle has settled yet: step = -1 in every record
+        HIPCHECK(c, dalloc(&c->d_settle, n));
+        HIPCHECK(c, hipMemsetAsync(c->d_settle, 0xff, sizeof(SettleRec) * std::max<int64_t>(n, 1), c->stream));
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->cb = 0;
+    c->latest = 0;
+    c->wslot = 0;
+    c->step = 0;
+    c->last_sort = 0;
+    return START_OK;
+}
+
+// hit = -1, load indices 0..n-1 and the live count of a new population.
+int init_population_tail(start_ctx* c, int64_t n)
+{
+    std::vector<uint32_t> pidx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) pidx[(size_t)i] = (uint32_t)i;
+    std::vector<int32_t> hit((size_t)n, -1);
+    HIPCHECK(c, hipMemcpy(c->A.hit, hit.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->A.pidx, pidx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    int64_t nact = n;
+    HIPCHECK(c, hipMemcpy(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice));
+    return START_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -842,65 +912,12 @@ int start_load_batch(start_ctx* c, int32_t n_runs, const start_run_desc* runs, c
             }
         }
     }
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    free_particles(c);
-    c->runs = rk;
-    int rc = upload_runs(c);
+    int rc = begin_population(c, rk, n, ids);
+    if (rc != START_OK || n == 0) return rc;
+    rc = init_population_tail(c, n);
     if (rc != START_OK) return rc;
-    rc = alloc_run_outputs(c);
-    if (rc != START_OK) return rc;
-    hipError_t he = alloc_pbuf(c->A, n, c->es);
-    if (he == hipSuccess) he = dalloc(&c->d_wl, n);
-    if (he == hipSuccess) he = alloc_pbuf(c->B, n, c->es);
-    if (he == hipSuccess) he = dalloc(&c->d_keys, n);
-    if (he == hipSuccess) he = dalloc(&c->d_keys_out, n);
-    if (he == hipSuccess) he = dalloc(&c->d_idx, n);
-    if (he == hipSuccess) he = dalloc(&c->d_perm, n);
-    if (he == hipSuccess) {
-        int bits = 1;
-        while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
-        c->sort_bits = bits;
-        c->sort_temp = sort_temp_bytes(n, bits);
-        he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
-    }
-    if (he != hipSuccess) {
-        free_particles(c);
-        return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
-                    std::string("particle allocation failed: ") + hipGetErrorString(he));
-    }
-    c->n = n;
-    c->ids.assign(ids ? ids : (const uint64_t*)nullptr, ids ? ids + n : (const uint64_t*)nullptr);
-    if (!ids) {
-        c->ids.resize((size_t)n);
-        for (int64_t i = 0; i < n; ++i) c->ids[(size_t)i] = (uint64_t)i;
-    }
-    const int64_t CL = c->charge_len();
-    HIPCHECK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned long long) * kCntTotal, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_tri_count, 0,
-                               sizeof(unsigned long long) * c->n_runs() * std::max<int64_t>(c->hm.n_tris, 1),
-                               c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_node_charge, 0, sizeof(double) * 2 * CL, c->stream));
-    if (c->record) {  // no particle has settled yet: step = -1 in every record
-        HIPCHECK(c, dalloc(&c->d_settle, n));
-        HIPCHECK(c, hipMemsetAsync(c->d_settle, 0xff, sizeof(SettleRec) * std::max<int64_t>(n, 1), c->stream));
-    }
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    c->cb = 0;
-    c->latest = 0;
-    c->wslot = 0;
-    c->step = 0;
-    c->last_sort = 0;
-    if (n == 0) return START_OK;
-
-    std::vector<uint32_t> pidx((size_t)n);
-    for (int64_t i = 0; i < n; ++i) pidx[(size_t)i] = (uint32_t)i;
-    std::vector<int32_t> hit((size_t)n, -1);
     size_t b8 = sizeof(double) * (size_t)n;
     HIPCHECK(c, hipMemcpy(c->A.flags, flags.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->A.hit, hit.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->A.pidx, pidx.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-    int64_t nact = n;
-    HIPCHECK(c, hipMemcpy(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice));
     DevMesh M = c->mesh_view();
     if (!c->mixed) {
         HIPCHECK(c, hipMemcpy(c->A.x, x, b8, hipMemcpyHostToDevice));
@@ -941,6 +958,145 @@ int start_load_batch(start_ctx* c, int32_t n_runs, const start_run_desc* runs, c
     if (c->cfg.sort_interval > 0) {
         int rc2 = sort_now(c);
         if (rc2 != START_OK) return rc2;
+    }
+    return check_kernel_errors(c);
+}
+
+int start_generate(start_ctx* c, int32_t n_sources, const start_source_desc* src, uint64_t seed)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (n_sources < 1 || !src) return fail(c, START_ERR_ARG, "no particle sources");
+    std::vector<SourceK> sk((size_t)n_sources);
+    int64_t n = 0, n_cells = 0;
+    for (int32_t s = 0; s < n_sources; ++s) {
+        const start_source_desc& d = src[s];
+        const std::string where = "source " + std::to_string(s) + ": ";
+        if (d.kind != START_SOURCE_POINT && d.kind != START_SOURCE_SURFACE)
+            return fail(c, START_ERR_ARG, where + "unknown source kind");
+        if (d.species < 0 || d.species >= (int32_t)c->species.size())
+            return fail(c, START_ERR_ARG, where + "species index out of range");
+        if (d.count <= 0)  // ParticleGeneratorHost.cpp:55-59
+            return fail(c, START_ERR_ARG, where + "there is no need to generate 0 objects");
+        if (d.energy < START_ENERGY_MONO || d.energy > START_ENERGY_LOGNORMAL)
+            return fail(c, START_ERR_ARG, where + "unknown energy distribution");
+        if (d.angle < START_ANGLE_FAN || d.angle > START_ANGLE_CONE)
+            return fail(c, START_ERR_ARG, where + "unknown direction distribution");
+        if (d.energy == START_ENERGY_MONO && d.energy_eV == 0.0)  // ParticleGeneratorHost.cpp:61-66
+            return fail(c, START_ERR_ARG, where + "source with zero energy");
+        if (d.energy == START_ENERGY_MAXWELLIAN && !(d.p0 > 0.0))
+            return fail(c, START_ERR_ARG, where + "Maxwellian temperature must be > 0");
+        if (d.energy == START_ENERGY_THOMPSON && !(d.p0 > 0.0 && d.p1 > 0.0))
+            return fail(c, START_ERR_ARG, where + "Thompson U_b and E_max must be > 0");
+        if (d.energy == START_ENERGY_LOGNORMAL && !(d.p0 > 0.0 && d.p1 >= 0.0))
+            return fail(c, START_ERR_ARG, where + "lognormal mean must be > 0 and sigma >= 0");
+        SourceK& k = sk[(size_t)s];
+        k.kind = d.kind;
+        k.energy = d.energy;
+        k.angle = d.angle;
+        k.flags = d.species << kSpeciesShift;
+        k.first = n;
+        k.count = d.count;
+        k.energy_eV = d.energy_eV;
+        k.p0 = d.p0;
+        k.p1 = d.p1;
+        k.mass = c->species[(size_t)d.species].mass;
+        for (int q = 0; q < 3; ++q) k.pos[q] = d.position[q];
+        k.theta = d.theta;
+        k.phi = d.phi;
+        k.expansion = d.expansion;
+        k.displace = d.displace;
+        k.cell0 = n_cells;
+        k.n_cells = 0;
+        if (d.kind == START_SOURCE_SURFACE) {
+            if (!d.centres || !d.normals || d.n_cells <= 0)  // ParticleGeneratorHost.cpp:103-107
+                return fail(c, START_ERR_ARG, where + "base coordinates for surface source are empty");
+            for (int64_t j = 0; j < d.n_cells; ++j) {
+                const double* nn = d.normals + 3 * j;
+                if (nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2] == 0.0)  // :130-134
+                    return fail(c, START_ERR_ARG, where + "normal vector magnitude is zero");
+            }
+            k.n_cells = d.n_cells;
+            n_cells += d.n_cells;
+        }
+        n += d.count;
+    }
+    if (n > (int64_t)0x7fffffff) return fail(c, START_ERR_ARG, "particle count out of range");
+    // Cells of every surface source in one array, each source's cells in a
+    // seeded random order: the first count % n_cells of them take one particle
+    // more, as the reference hands the remainder to randomly shuffled cells
+    // (ParticleGeneratorHost.cpp:109-118).  Fisher-Yates with Philox variates
+    // (counter {k, 2, source, 0}).
+    std::vector<double> cen((size_t)std::max<int64_t>(3 * n_cells, 3)), nor(cen.size());
+    for (int32_t s = 0; s < n_sources; ++s) {
+        const SourceK& k = sk[(size_t)s];
+        if (k.n_cells == 0) continue;
+        std::vector<int64_t> order((size_t)k.n_cells);
+        for (int64_t j = 0; j < k.n_cells; ++j) order[(size_t)j] = j;
+        for (int64_t j = k.n_cells - 1; j > 0; --j) {
+            uint32_t w[4];
+            philox4x32_10((uint32_t)j, 2u, (uint32_t)s, 0u, (uint32_t)seed, (uint32_t)(seed >> 32), w);
+            const double u = (double)(((uint64_t)w[0] << 21) | (w[1] >> 11)) * (1.0 / 9007199254740992.0);
+            const int64_t r = std::min<int64_t>((int64_t)(u * (double)(j + 1)), j);
+            std::swap(order[(size_t)j], order[(size_t)r]);
+        }
+        for (int64_t j = 0; j < k.n_cells; ++j)
+            for (int q = 0; q < 3; ++q) {
+                cen[(size_t)(3 * (k.cell0 + j) + q)] = src[s].centres[3 * order[(size_t)j] + q];
+                nor[(size_t)(3 * (k.cell0 + j) + q)] = src[s].normals[3 * order[(size_t)j] + q];
+            }
+    }
+    std::vector<RunK> rk(1);
+    rk[0].conc = c->prm.gas.concentration;
+    rk[0].s0 = 1.0;
+    rk[0].s1 = 0.0;
+    rk[0].key = c->prm.run;
+    rk[0].first = 0;
+    rk[0].node_base = 0;
+    rk[0].tri_base = 0;
+    int rc = begin_population(c, rk, n, nullptr);
+    if (rc != START_OK) return rc;
+    rc = init_population_tail(c, n);
+    if (rc != START_OK) return rc;
+    double *d_cen = nullptr, *d_nor = nullptr, *t[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipError_t e = dalloc(&d_cen, cen.size());
+    if (e == hipSuccess) e = dalloc(&d_nor, nor.size());
+    if (e == hipSuccess) e = hipMemcpy(d_cen, cen.data(), sizeof(double) * cen.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_nor, nor.data(), sizeof(double) * nor.size(), hipMemcpyHostToDevice);
+    // FP64 state: drawn in place; mixed: drawn into FP64 scratch, then positions
+    // to the owner's frame (by the locate) and velocities to FP32
+    double* dst[6] = {(double*)c->A.x, (double*)c->A.y, (double*)c->A.z,
+                      (double*)c->A.vx, (double*)c->A.vy, (double*)c->A.vz};
+    if (c->mixed)
+        for (int q = 0; q < 6 && e == hipSuccess; ++q) {
+            e = dalloc(&t[q], n);
+            dst[q] = t[q];
+        }
+    if (e == hipSuccess) {
+        for (int32_t s = 0; s < n_sources; ++s)
+            launch_generate(c->stream, sk[(size_t)s], (uint32_t)s, seed, d_cen, d_nor, dst[0], dst[1], dst[2],
+                            dst[3], dst[4], dst[5], c->A.flags);
+        DevMesh M = c->mesh_view();
+        if (!c->mixed) {
+            launch_locate(c->stream, M, dst[0], dst[1], dst[2], n, c->A.tet, c->A.flags, nullptr, nullptr, nullptr,
+                          c->d_cnt);
+        } else {
+            launch_locate(c->stream, M, dst[0], dst[1], dst[2], n, c->A.tet, c->A.flags, (float*)c->A.x,
+                          (float*)c->A.y, (float*)c->A.z, c->d_cnt);
+            launch_to_f32(c->stream, dst[3], (float*)c->A.vx, n);
+            launch_to_f32(c->stream, dst[4], (float*)c->A.vy, n);
+            launch_to_f32(c->stream, dst[5], (float*)c->A.vz, n);
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(d_cen);
+    (void)hipFree(d_nor);
+    for (double* p : t)
+        if (p) (void)hipFree(p);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("particle sources: ") + hipGetErrorString(e));
+    if (c->cfg.sort_interval > 0) {
+        rc = sort_now(c);
+        if (rc != START_OK) return rc;
     }
     return check_kernel_errors(c);
 }

@@ -93,6 +93,42 @@ class ParticleInCellStep:
         self.n = len(pos)
         self.n_runs = 1
 
+    def generate(self, sources: list[dict], seed: int = 0):
+        """Particles drawn on the device from point / surface sources (replaces
+        ParticleGeneratorHost / ParticleGeneratorDevice and the displacer).
+        Each source: kind ("point" | "surface"), species, count, energy ("mono" |
+        "maxwellian" | "thompson" | "lognormal"), angle ("fan" | "cosine" |
+        "isotropic" | "cone"), energy_eV, p0, p1, position, theta, phi,
+        expansion, displace, and for surface sources centres and normals
+        (n_cells x 3)."""
+        descs = (_lib.SourceDesc * len(sources))()
+        keep = []
+        n = 0
+        for d, s in zip(descs, sources):
+            d.kind = _lib.SOURCE[s.get("kind", "point")]
+            d.species = int(s.get("species", 0))
+            d.count = int(s["count"])
+            d.energy = _lib.ENERGY[s.get("energy", "mono")]
+            d.angle = _lib.ANGLE[s.get("angle", "fan")]
+            d.energy_eV = float(s.get("energy_eV", 0.0))
+            d.p0 = float(s.get("p0", 0.0))
+            d.p1 = float(s.get("p1", 0.0))
+            for q, v in enumerate(s.get("position", (0.0, 0.0, 0.0))):
+                d.position[q] = float(v)
+            d.theta = float(s.get("theta", 0.0))
+            d.phi = float(s.get("phi", 0.0))
+            d.expansion = float(s.get("expansion", 0.0))
+            d.displace = float(s.get("displace", 0.0))
+            if d.kind == _lib.SOURCE["surface"]:
+                cen = np.ascontiguousarray(s["centres"], dtype=np.float64).reshape(-1, 3)
+                nor = np.ascontiguousarray(s["normals"], dtype=np.float64).reshape(-1, 3)
+                keep += [cen, nor]
+                d.centres, d.normals, d.n_cells = _dp(cen), _dp(nor), len(cen)
+            n += d.count
+        self._check(_lib.lib().start_generate(self.ctx, len(sources), descs, int(seed)))
+        self.n = n
+        self.n_runs = 1
+
     def load_batch(self, runs: list[dict], pos: np.ndarray, vel: np.ndarray, species: np.ndarray, ids=None):
         """Independent runs on the shared mesh, their rows consecutive in run
         order.  Each run: {"n": rows, "scale": (s0, s1), "concentration": n,

@@ -12,10 +12,22 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    src = open(os.path.join(ROOT, "include", "start_gpu.h")).read()
+def _declared(header="start_gpu.h"):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(start_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_every_header_is_checked():
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["start_gpu.h", "start_mesh.h"]
+
+
+def test_mesh_library_exports_every_declared_symbol():
+    lib = C.CDLL(os.path.join(ROOT, "start_amd", "libstart_mesh.so"))
+    names = _declared("start_mesh.h")
+    assert names == ["start_gmsh_free", "start_gmsh_last_error", "start_gmsh_read"]
+    for name in names:
+        assert hasattr(lib, name), name
 
 
 def test_header_declares_entry_points():

@@ -228,6 +228,30 @@ int start_get_launch(start_ctx* ctx, start_launch_cfg* cfg);
  * state, and sets the fastest.  best and ms_per_step may be NULL. */
 int start_autotune(start_ctx* ctx, int32_t trial_steps, start_launch_cfg* best, double* ms_per_step);
 
+/* ---- particle movements (the particleMovementMap argument of process) ----
+ * Replaces ParticleMovementTracker::recordMovement
+ * (ParticleMovementTracker.cpp:4-16) as process calls it: the position at the
+ * start of every step a particle is pushed and moved in
+ * (ParticleDynamicsProcessor.cpp:182-184), and the point where a settling
+ * segment meets its triangle (ParticleSurfaceCollisionHandler.cpp:91-99,
+ * :134-138), computed as in start_read_hits.
+ *
+ * start_track_movements(ctx, max_particles, max_points) starts recording for
+ * the first max_particles particles, in load order, that the next step moves
+ * (not settled, |v| >= 10 m/s, inside the expanded bounding box): the map
+ * of the reference fills in its loop order while it holds fewer than
+ * maxParticles (200,000 by default, ParticleMovementTracker.hpp:43), and no
+ * particle starts moving later in the loop.  Each slot keeps its first
+ * max_points points; later points are counted, not kept.  max_particles = 0
+ * stops recording.  Loading or generating particles stops it too.
+ * start_read_movements: with ids, counts and xyz all null, *n = the number of
+ * tracked particles; otherwise (*n in = capacity) ids[s], counts[s] (points
+ * recorded, possibly above max_points) and xyz[3 * (s * max_points + k)] for
+ * slot s.  The JSON of saveMovementsToJson is written on the host from these
+ * (start_write_movements_json, include/start_output.h). */
+int start_track_movements(start_ctx* ctx, int64_t max_particles, int32_t max_points);
+int start_read_movements(start_ctx* ctx, int64_t* n, uint64_t* ids, int32_t* counts, double* xyz);
+
 /* ---- adjoint of the collisionless push and the surface-hit map ----
  * (no counterpart in the reference; gives d(deposition moments)/d(electrode
  * potentials) for parameter optimisation).  The field of the run is

@@ -1,4 +1,4 @@
-/* Gmsh 4.1 mesh reader (host side, libstart_mesh.so; no GPU needed).
+/* Gmsh 4.1 mesh reader (host side, libstart_io.so; no GPU needed).
  *
  * Replaces the reference's libgmsh reads: VolumetricMesh::_readMesh
  * (VolumetricMesh.cpp:35-121: every 4-node tetrahedron, element type 4, with

@@ -78,6 +78,8 @@ def lib() -> C.CDLL:
         L.or_step.argtypes = [vp, P(Params), P(SpeciesK), P(d), i64] + [P(d)] * 6 + \
             [P(i32), P(i32), P(i32), P(i32), P(u64), u32, P(u64), P(d), C.c_int, P(u64)]
         L.or_step.restype = C.c_int
+        L.or_step_moved.argtypes = L.or_step.argtypes + [P(C.c_uint8)]
+        L.or_step_moved.restype = C.c_int
         L.or_scatter_charge.argtypes = [vp, P(SpeciesK), i64, P(d), P(d), P(d), P(i32), P(i32),
                                         P(i32), P(d)]
         L.or_tet_volumes.argtypes = [vp, P(d)]
@@ -256,15 +258,19 @@ class Sim:
                           model, int(em_push), len(species_k))
         self.tri_count = np.zeros(max(mesh.n_tris, 1), dtype=np.uint64)
 
-    def step(self, st: State, step: int, node_charge: np.ndarray | None = None, nthreads: int = 1) -> int:
+    def step(self, st: State, step: int, node_charge: np.ndarray | None = None, nthreads: int = 1,
+             moved: np.ndarray | None = None) -> int:
+        """One step; with `moved` (uint8, one per particle) also the particles
+        whose start position the step records as a movement."""
         processed = C.c_uint64(0)
         nc = _p(node_charge, C.c_double) if node_charge is not None else None
-        lib().or_step(self.mesh.h, C.byref(self.prm), self.sp, _p(self.E, C.c_double), len(st.x),
-                      _p(st.x, C.c_double), _p(st.y, C.c_double), _p(st.z, C.c_double),
-                      _p(st.vx, C.c_double), _p(st.vy, C.c_double), _p(st.vz, C.c_double),
-                      _p(st.tet, C.c_int32), _p(st.status, C.c_int32), _p(st.hit, C.c_int32),
-                      _p(st.species, C.c_int32), _p(st.pid, C.c_uint64), step,
-                      _p(self.tri_count, C.c_uint64), nc, nthreads, C.byref(processed))
+        mv = _p(moved, C.c_uint8) if moved is not None else None
+        lib().or_step_moved(self.mesh.h, C.byref(self.prm), self.sp, _p(self.E, C.c_double), len(st.x),
+                            _p(st.x, C.c_double), _p(st.y, C.c_double), _p(st.z, C.c_double),
+                            _p(st.vx, C.c_double), _p(st.vy, C.c_double), _p(st.vz, C.c_double),
+                            _p(st.tet, C.c_int32), _p(st.status, C.c_int32), _p(st.hit, C.c_int32),
+                            _p(st.species, C.c_int32), _p(st.pid, C.c_uint64), step,
+                            _p(self.tri_count, C.c_uint64), nc, nthreads, C.byref(processed), mv)
         return processed.value
 
     def scatter_charge(self, st: State, node_charge: np.ndarray):

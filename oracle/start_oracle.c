@@ -792,13 +792,15 @@ void or_node_density(const or_mesh* m, const or_species_k* sp, int64_t n, const 
 /* ------------------------------------------------------------------------- */
 /* One step over all particles (ParticleDynamicsProcessor.cpp:161-200), then  */
 /* relocation and charge for the next step's gather                          */
-/* (NodeChargeDensityProcessor.cpp:197-220).                                 */
+/* (NodeChargeDensityProcessor.cpp:197-220).  With `moved`, moved[i] = 1 for  */
+/* the particles that reach the movement record after the push               */
+/* (ParticleDynamicsProcessor.cpp:182-184), 0 for the others.                 */
 /* ------------------------------------------------------------------------- */
-int or_step(const or_mesh* m, const or_params* prm, const or_species_k* sp, const double* E,
-            int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
-            int32_t* tet, int32_t* status, int32_t* hit, const int32_t* species,
-            const uint64_t* pid, uint32_t step, uint64_t* tri_count, double* node_charge,
-            int nthreads, uint64_t* processed)
+int or_step_moved(const or_mesh* m, const or_params* prm, const or_species_k* sp, const double* E,
+                  int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                  int32_t* tet, int32_t* status, int32_t* hit, const int32_t* species,
+                  const uint64_t* pid, uint32_t step, uint64_t* tri_count, double* node_charge,
+                  int nthreads, uint64_t* processed, uint8_t* moved)
 {
     uint64_t nproc = 0;
 #ifdef _OPENMP
@@ -808,6 +810,7 @@ int or_step(const or_mesh* m, const or_params* prm, const or_species_k* sp, cons
 #endif
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : nproc)
     for (int64_t i = 0; i < n; ++i) {
+        if (moved) moved[i] = 0;
         /* 1. settled particles are skipped (ParticleDynamicsProcessor.cpp:167-168) */
         if (status[i] == OR_SETTLED) continue;
         double v[3] = {vx[i], vy[i], vz[i]};
@@ -825,6 +828,8 @@ int or_step(const or_mesh* m, const or_params* prm, const or_species_k* sp, cons
             const or_species_k* k = &sp[species[i]];
             /* 4. push with the owning tet's field (ParticleDynamicsProcessor.cpp:179-180) */
             if (prm->em_push && t0 >= 0) or_push(E + 3 * (int64_t)t0, k->charge, k->mass, prm->dt, v);
+            /* 5. the movement record of the position before the move (:182-184) */
+            if (moved) moved[i] = 1;
             /* 6. move (Particle.cpp:157-165) */
             double p1[3] = {p0[0] + v[0] * prm->dt, p0[1] + v[1] * prm->dt, p0[2] + v[2] * prm->dt};
             /* 6b. degenerate segment: nothing else this step (:188-190) */
@@ -869,4 +874,14 @@ int or_step(const or_mesh* m, const or_params* prm, const or_species_k* sp, cons
     }
     if (processed) *processed = nproc;
     return 0;
+}
+
+int or_step(const or_mesh* m, const or_params* prm, const or_species_k* sp, const double* E,
+            int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+            int32_t* tet, int32_t* status, int32_t* hit, const int32_t* species,
+            const uint64_t* pid, uint32_t step, uint64_t* tri_count, double* node_charge,
+            int nthreads, uint64_t* processed)
+{
+    return or_step_moved(m, prm, sp, E, n, x, y, z, vx, vy, vz, tet, status, hit, species, pid, step,
+                         tri_count, node_charge, nthreads, processed, NULL);
 }

@@ -106,6 +106,13 @@ int or_step(const or_mesh* m, const or_params* prm, const or_species_k* sp, cons
             int32_t* tet, int32_t* status, int32_t* hit, const int32_t* species,
             const uint64_t* pid, uint32_t step, uint64_t* tri_count, double* node_charge,
             int nthreads, uint64_t* processed);
+/* or_step, and moved[i] = 1 for the particles whose position the step records
+ * as a movement (pushed and moved this step), 0 for the others */
+int or_step_moved(const or_mesh* m, const or_params* prm, const or_species_k* sp, const double* E,
+                  int64_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                  int32_t* tet, int32_t* status, int32_t* hit, const int32_t* species,
+                  const uint64_t* pid, uint32_t step, uint64_t* tri_count, double* node_charge,
+                  int nthreads, uint64_t* processed, uint8_t* moved);
 
 void or_scatter_charge(const or_mesh* m, const or_species_k* sp, int64_t n, const double* x,
                        const double* y, const double* z, const int32_t* tet, const int32_t* status,

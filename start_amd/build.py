@@ -45,14 +45,14 @@ def build_library(verbose: bool = False) -> str:
 
 HOST = os.path.join(HERE, "host")
 HOST_LIB = os.path.join(HERE, "libstart_host.so")
-MESH_LIB = os.path.join(HERE, "libstart_mesh.so")
+IO_LIB = os.path.join(HERE, "libstart_io.so")
 SHIM_CHECK = os.path.join(HERE, "shim_check")
 
 
 def build_host(verbose: bool = False) -> list[str]:
     """The C++ step-loop entry points (libstart_host.so) over libstart_gpu.so,
     the shim_check driver that runs them like the reference's main loop, and
-    the Gmsh 4.1 reader (libstart_mesh.so, host only)."""
+    the Gmsh 4.1 reader and the output writers (libstart_io.so, host only)."""
     gxx = shutil.which("g++") or "g++"
     common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-I", HOST]
     cmds = [
@@ -60,10 +60,10 @@ def build_host(verbose: bool = False) -> list[str]:
                           "-L", HERE, "-lstart_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"],
         [gxx] + common + ["-o", SHIM_CHECK + ".tmp", os.path.join(HOST, "shim_check.cpp"),
                           "-L", HERE, "-lstart_host", "-lstart_gpu", "-Wl,-rpath,$ORIGIN"],
-        [gxx] + common + ["-fPIC", "-shared", "-o", MESH_LIB + ".tmp", os.path.join(HOST, "GmshReader.cpp"),
-                          "-Wl,--no-undefined"],
+        [gxx] + common + ["-fPIC", "-shared", "-o", IO_LIB + ".tmp", os.path.join(HOST, "GmshReader.cpp"),
+                          os.path.join(HOST, "Outputs.cpp"), "-Wl,--no-undefined"],
     ]
-    outs = [HOST_LIB, SHIM_CHECK, MESH_LIB]
+    outs = [HOST_LIB, SHIM_CHECK, IO_LIB]
     for cmd, out in zip(cmds, outs):
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -127,6 +127,13 @@ struct StepParams {
     int32_t work_slot;        // slot of this launch's work-list, step and tile counters
     int32_t nsteps;           // steps fused in this launch (charge accumulates over them)
     SettleRec* settle;        // null, or one record per load index
+    // movement records (ParticleMovementTracker.cpp:4-16): null, or the slot
+    // of each load index (-1: not tracked), points recorded per slot, and
+    // trk_max absolute points per slot
+    const int32_t* trk_slot;
+    int32_t* trk_count;
+    double* trk_xyz;
+    int32_t trk_max;
 };
 
 }  // namespace sg

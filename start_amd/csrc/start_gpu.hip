@@ -39,6 +39,9 @@ template <typename R>
 void launch_adjoint_push(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,
                          const RunK* runs, double dt, int32_t k0, const SettleRec* rec, const double* lambda,
                          double* grad, unsigned long long* cnt);
+template <typename R>
+void launch_track_eligible(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const StepParams& S,
+                           uint8_t* out);
 void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32);
 void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
@@ -147,6 +150,13 @@ struct start_ctx {
     // settle records for the adjoint (by load index), when recording
     SettleRec* d_settle = nullptr;
     bool record = false;
+    // movement records: slot of each load index, points per slot, the points
+    // (trk_max per slot), and the load index of each slot
+    int32_t* d_trk_slot = nullptr;
+    int32_t* d_trk_count = nullptr;
+    double* d_trk_xyz = nullptr;
+    int32_t trk_max = 0;
+    std::vector<int64_t> trk_rows;
     // captured step chunks
     std::vector<GraphEntry> graphs;
     // timing
@@ -257,9 +267,21 @@ void drop_graphs(start_ctx* c)
     c->graphs.clear();
 }
 
+void free_tracking(start_ctx* c)
+{
+    void* ptrs[] = {c->d_trk_slot, c->d_trk_count, c->d_trk_xyz};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    c->d_trk_slot = c->d_trk_count = nullptr;
+    c->d_trk_xyz = nullptr;
+    c->trk_max = 0;
+    c->trk_rows.clear();
+}
+
 void free_particles(start_ctx* c)
 {
     drop_graphs(c);
+    free_tracking(c);
     free_pbuf(c->A);
     free_pbuf(c->B);
     void* ptrs[] = {c->d_wl, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp, c->d_settle};
@@ -465,6 +487,10 @@ StepParams step_params(const start_ctx* c)
     S.n_active = c->d_nactive;
     S.charge_len = c->charge_len();
     S.settle = c->record ? c->d_settle : nullptr;
+    S.trk_slot = c->d_trk_slot;
+    S.trk_count = c->d_trk_count;
+    S.trk_xyz = c->d_trk_xyz;
+    S.trk_max = c->trk_max;
     return S;
 }
 
@@ -1287,6 +1313,9 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
         c->timing = timing;
         return rc;
     }
+    // the trials record no movements (the restored state does not undo them)
+    int32_t* const trk_keep = c->d_trk_slot;
+    c->d_trk_slot = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -1335,6 +1364,7 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->timing = timing;
+    c->d_trk_slot = trk_keep;
     c->cfg = rc == START_OK ? best_cfg : keep;
     c->prm.steps_per_launch = c->cfg.steps_per_launch;
     c->prm.sort_interval = c->cfg.sort_interval;
@@ -1342,6 +1372,76 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
     if (rc2 != START_OK) return rc2;
     if (best) *best = best_cfg;
     if (ms_per_step) *ms_per_step = best_ms;
+    return check_kernel_errors(c);
+}
+
+int start_track_movements(start_ctx* c, int64_t max_particles, int32_t max_points)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (max_particles < 0 || max_points < 0 || (max_particles > 0 && max_points == 0))
+        return fail(c, START_ERR_ARG, "max_particles >= 0 and max_points >= 1 are required");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured launches hold the record pointers
+    free_tracking(c);
+    const int64_t n = c->n;
+    if (max_particles == 0 || n == 0) return START_OK;
+    // the first max_particles particles, in load order, that the next step
+    // records (the reference's map fills in its loop order while it has room)
+    uint8_t* d_el = nullptr;
+    HIPCHECK(c, dalloc(&d_el, n));
+    StepParams S = step_params(c);
+    if (c->mixed) launch_track_eligible<float>(c->stream, c->mesh_view(), c->view<float>(), S, d_el);
+    else launch_track_eligible<double>(c->stream, c->mesh_view(), c->view<double>(), S, d_el);
+    std::vector<uint8_t> el(n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(el.data(), d_el, (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_el);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("movement tracking: ") + hipGetErrorString(e));
+    std::vector<int32_t> slot(n, -1);
+    for (int64_t r = 0; r < n && (int64_t)c->trk_rows.size() < max_particles; ++r)
+        if (el[r]) {
+            slot[r] = (int32_t)c->trk_rows.size();
+            c->trk_rows.push_back(r);
+        }
+    const int64_t m = (int64_t)c->trk_rows.size();
+    if (m == 0) return START_OK;
+    e = dalloc(&c->d_trk_slot, n);
+    if (e == hipSuccess) e = dalloc(&c->d_trk_count, m);
+    if (e == hipSuccess) e = dalloc(&c->d_trk_xyz, 3 * (size_t)m * (size_t)max_points);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->d_trk_slot, slot.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(c->d_trk_count, 0, sizeof(int32_t) * (size_t)m);
+    if (e != hipSuccess) {
+        free_tracking(c);
+        return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("movement tracking: ") + hipGetErrorString(e));
+    }
+    c->trk_max = max_points;
+    return START_OK;
+}
+
+int start_read_movements(start_ctx* c, int64_t* n, uint64_t* ids, int32_t* counts, double* xyz)
+{
+    if (!c || !n) return fail(c, START_ERR_ARG, "null argument");
+    const int64_t m = (int64_t)c->trk_rows.size();
+    if (!ids && !counts && !xyz) {
+        *n = m;
+        return START_OK;
+    }
+    if (*n < m) return fail(c, START_ERR_ARG, "capacity below the number of tracked particles");
+    *n = m;
+    if (m == 0) return START_OK;
+    if (ids)
+        for (int64_t s = 0; s < m; ++s) ids[s] = c->ids[c->trk_rows[s]];
+    hipError_t e = hipSuccess;
+    if (counts)
+        e = hipMemcpyAsync(counts, c->d_trk_count, sizeof(int32_t) * (size_t)m, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && xyz)
+        e = hipMemcpyAsync(xyz, c->d_trk_xyz, sizeof(double) * 3 * (size_t)m * (size_t)c->trk_max,
+                           hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("movements: ") + hipGetErrorString(e));
     return check_kernel_errors(c);
 }
 

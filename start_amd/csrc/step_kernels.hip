@@ -748,6 +748,71 @@ __device__ __noinline__ void record_settle(SettleRec* rec, uint32_t pidx, uint32
     rec[pidx] = r;
 }
 
+// Where the segment xa -> xa + D meets the plane of triangle `tri`:
+// xa + s D with s = n.(A - xa) / n.D, n = (B - A) x (C - A).
+__device__ inline void hit_plane(const DevMesh& M, int32_t tri, const double* xa, const double* D, double* n,
+                                 double& s)
+{
+    double A[3], B[3], C[3];
+    load_node(M, M.tris[3 * (int64_t)tri], A);
+    load_node(M, M.tris[3 * (int64_t)tri + 1], B);
+    load_node(M, M.tris[3 * (int64_t)tri + 2], C);
+    double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    double w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    n[0] = u[1] * w[2] - u[2] * w[1];
+    n[1] = u[2] * w[0] - u[0] * w[2];
+    n[2] = u[0] * w[1] - u[1] * w[0];
+    double nD = (n[0] * D[0] + n[1] * D[1]) + n[2] * D[2];
+    double nA = (n[0] * (A[0] - xa[0]) + n[1] * (A[1] - xa[1])) + n[2] * (A[2] - xa[2]);
+    s = nA / nD;
+}
+
+// Movement records of the tracked particles (ParticleMovementTracker.cpp:4-16):
+// one absolute point appended to the slot of load index li, if it has one.
+// Points beyond the slot's capacity are counted, not stored.  One thread owns
+// a slot at a time (the particle's lane in the step kernels), so no atomics.
+// The callers test S.trk_slot (kernel-uniform) and look the slot up here, so
+// that with recording off no register is held across the step loop.
+__device__ __noinline__ void track_point(const int32_t* slot_of, int32_t* cnt, double* xyz, int32_t max_pts,
+                                         uint32_t li, double x, double y, double z)
+{
+    const int32_t slot = slot_of[li];
+    if (slot < 0) return;
+    const int32_t c = cnt[slot];
+    if (c < max_pts) {
+        double* q = xyz + 3 * ((int64_t)slot * max_pts + c);
+        q[0] = x;
+        q[1] = y;
+        q[2] = z;
+    }
+    cnt[slot] = c + 1;
+}
+
+// the position at the start of a step (recorded after the push, before the
+// move: ParticleDynamicsProcessor.cpp:182-184)
+template <typename R>
+__device__ __forceinline__ void track_start(const DevMesh& M, const StepParams& S, uint32_t li, int32_t t, R x, R y,
+                                            R z)
+{
+    double a[3];
+    abs_pos(M, t, x, y, z, a);
+    track_point(S.trk_slot, S.trk_count, S.trk_xyz, S.trk_max, li, a[0], a[1], a[2]);
+}
+
+// the point where the settling segment xa -> (x1 in the frame of t) meets its
+// triangle (ParticleSurfaceCollisionHandler.cpp:91-99, :134-138)
+template <typename R>
+__device__ __forceinline__ void track_hit(const DevMesh& M, const StepParams& S, uint32_t li, const double* xa,
+                                          int32_t t, R x1, R y1, R z1, int32_t tri)
+{
+    double b[3], D[3], n[3], s;
+    abs_pos(M, t, x1, y1, z1, b);
+    for (int k = 0; k < 3; ++k) D[k] = b[k] - xa[k];
+    hit_plane(M, tri, xa, D, n, s);
+    track_point(S.trk_slot, S.trk_count, S.trk_xyz, S.trk_max, li, xa[0] + s * D[0], xa[1] + s * D[1],
+                xa[2] + s * D[2]);
+}
+
 template <typename R>
 struct PState {
     R x, y, z, vx, vy, vz;
@@ -1049,15 +1114,17 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
                 if (r.outcome == kFrozenNow) break;
                 ++processed;
                 moved = true;
+                if (S.trk_slot) track_start<R>(M, S, P.pidx[i], t_before, px, py, pz);
                 if (r.outcome == kSettledNow) {
                     s.f = (s.f | kSettled) & ~kStrict;
                     P.hit[i] = r.hit_tri;
                     atomicAdd(&tri_count[rk->tri_base + r.hit_tri], 1ull);
                     atomicAdd(&cnt[kCntSettled], 1ull);
-                    if (S.settle) {
+                    if (S.settle || S.trk_slot) {
                         double a[3];
                         abs_pos(M, t_before, px, py, pz, a);
-                        record_settle(S.settle, P.pidx[i], step0 + (uint32_t)ks, a[0], a[1], a[2]);
+                        if (S.settle) record_settle(S.settle, P.pidx[i], step0 + (uint32_t)ks, a[0], a[1], a[2]);
+                        if (S.trk_slot) track_hit<R>(M, S, P.pidx[i], a, s.t, s.x, s.y, s.z, r.hit_tri);
                     }
                     break;
                 }
@@ -1235,7 +1302,9 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
         int exh = 0, err = 0, nexact = 0;
         int32_t hit_tri = -1;
         double x0[3] = {it.x0, it.y0, it.z0};
+        // (the step's start was recorded by the streaming kernel)
         bool settled = finish_exact(M, S, k, x0, s, sink, hit_tri, exh, err);
+        if (settled && S.trk_slot) track_hit<R>(M, S, P.pidx[i], x0, s.t, s.x, s.y, s.z, hit_tri);
         uint32_t settle_step = step0 + (uint32_t)it.k;  // step and start of the settling segment
         double sx = x0[0], sy = x0[1], sz = x0[2];
         if (!settled) {
@@ -1247,6 +1316,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
                 StepResult rr = step_one<R, true, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (rr.outcome == kFrozenNow) break;
                 ++extra;
+                if (S.trk_slot) track_start<R>(M, S, P.pidx[i], t_before, px, py, pz);
                 if (rr.outcome == kDeferNow || rr.outcome == kSettledNow) {
                     double x0s[3];
                     abs_pos(M, t_before, px, py, pz, x0s);
@@ -1257,6 +1327,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
                         hit_tri = rr.hit_tri;
                         settled = true;
                     }
+                    if (settled && S.trk_slot) track_hit<R>(M, S, P.pidx[i], x0s, s.t, s.x, s.y, s.z, hit_tri);
                     settle_step = step0 + (uint32_t)ks;
                     sx = x0s[0];
                     sy = x0s[1];
@@ -1451,21 +1522,20 @@ __global__ void abs_positions_kernel(DevMesh M, const float* __restrict__ lx, co
 // The owners t_k are recomputed backwards -- x_{k-1} = x_k - v_k dt with v_k
 // the push of t_k undone -- instead of storing the trajectory.
 // ---------------------------------------------------------------------------
-__device__ inline void hit_plane(const DevMesh& M, int32_t tri, const double* xa, const double* D, double* n,
-                                 double& s)
+
+// Which particles the next step records a movement for, by load index: not
+// settled, |v| >= 10 m/s and inside the expanded bounding box (the filters of
+// ParticleDynamicsProcessor.cpp:166-176, as step_one applies them).
+template <typename R>
+__global__ void track_eligible_kernel(DevMesh M, DevParticles<R> P, StepParams S, uint8_t* __restrict__ out)
 {
-    double A[3], B[3], C[3];
-    load_node(M, M.tris[3 * (int64_t)tri], A);
-    load_node(M, M.tris[3 * (int64_t)tri + 1], B);
-    load_node(M, M.tris[3 * (int64_t)tri + 2], C);
-    double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
-    double w[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
-    n[0] = u[1] * w[2] - u[2] * w[1];
-    n[1] = u[2] * w[0] - u[0] * w[2];
-    n[2] = u[0] * w[1] - u[1] * w[0];
-    double nD = (n[0] * D[0] + n[1] * D[1]) + n[2] * D[2];
-    double nA = (n[0] * (A[0] - xa[0]) + n[1] * (A[1] - xa[1])) + n[2] * (A[2] - xa[2]);
-    s = nA / nD;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const int32_t f = P.flags[i];
+    const R vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+    const R vmod = m_sqrt<R>(vx * vx + vy * vy + vz * vz);
+    const bool live = !(f & kSettled) && !(vmod < R(10)) && in_bbox_at(M, S, P.tet[i], P.x[i], P.y[i], P.z[i]);
+    out[P.pidx[i]] = live ? 1 : 0;
 }
 
 // hit points, triangles and settle steps of the settled particles, in load order
@@ -1634,6 +1704,17 @@ void launch_hit_points(hipStream_t s, const DevMesh& M, const DevParticles<R>& P
 {
     if (P.n > 0) hit_points_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, rec, xh, tri, step);
 }
+
+template <typename R>
+void launch_track_eligible(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const StepParams& S,
+                           uint8_t* out)
+{
+    if (P.n > 0) track_eligible_kernel<R><<<blocks_for(P.n, 256), 256, 0, s>>>(M, P, S, out);
+}
+template void launch_track_eligible<double>(hipStream_t, const DevMesh&, const DevParticles<double>&,
+                                            const StepParams&, uint8_t*);
+template void launch_track_eligible<float>(hipStream_t, const DevMesh&, const DevParticles<float>&,
+                                           const StepParams&, uint8_t*);
 
 template <typename R>
 void launch_adjoint_push(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, const SpeciesK* sp,

@@ -205,6 +205,37 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_read_hits(self.ctx, _dp(xh), _ip(tri), _ip(step)))
         return xh, tri, step
 
+    def track_movements(self, max_particles: int = 200_000, max_points: int = 1000):
+        """Record the movements of the first `max_particles` moving particles
+        (load order): the start of every step and the surface hit point, up to
+        `max_points` each.  0 stops recording."""
+        self._max_points = int(max_points)
+        self._check(_lib.lib().start_track_movements(self.ctx, int(max_particles), int(max_points)))
+
+    def movements(self) -> dict:
+        """{particle id: (k, 3) array of recorded points} for the tracked
+        particles (the first max_points of each; see movement_counts)."""
+        L = _lib.lib()
+        n = C.c_int64(0)
+        self._check(L.start_read_movements(self.ctx, C.byref(n), None, None, None))
+        m, kmax = n.value, getattr(self, "_max_points", 0)
+        ids = np.empty(max(m, 1), dtype=np.uint64)
+        counts = np.empty(max(m, 1), dtype=np.int32)
+        xyz = np.empty(max(m * kmax * 3, 1), dtype=np.float64)
+        self._check(L.start_read_movements(self.ctx, C.byref(n), _up(ids), _ip(counts), _dp(xyz)))
+        xyz = xyz[: m * kmax * 3].reshape(m, kmax, 3)
+        return {int(ids[s]): xyz[s, : min(int(counts[s]), kmax)].copy() for s in range(m)}
+
+    def movement_counts(self) -> tuple[np.ndarray, np.ndarray]:
+        """(ids, points recorded) per tracked particle, counts possibly above max_points."""
+        L = _lib.lib()
+        n = C.c_int64(0)
+        self._check(L.start_read_movements(self.ctx, C.byref(n), None, None, None))
+        ids = np.empty(max(n.value, 1), dtype=np.uint64)
+        counts = np.empty(max(n.value, 1), dtype=np.int32)
+        self._check(L.start_read_movements(self.ctx, C.byref(n), _up(ids), _ip(counts), None))
+        return ids[: n.value], counts[: n.value]
+
     def adjoint_push(self, lam: np.ndarray, first_step: int = 0) -> np.ndarray:
         """dL/d(field scales) for lam = dL/d(hit point) per particle (n x 3, load order)."""
         lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)

@@ -1,4 +1,4 @@
-"""Gmsh 4.1 meshes through the C++ reader (libstart_mesh.so, include/start_mesh.h).
+"""Gmsh 4.1 meshes through the C++ reader (libstart_io.so, include/start_mesh.h).
 
 `read(path)` gives a `GmshMesh`: the `TetMesh` the engine takes (node rows in
 ascending tag order, tets and triangles as node rows) and the element tags
@@ -15,7 +15,7 @@ import numpy as np
 
 from .meshgen import TetMesh
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libstart_mesh.so")
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libstart_io.so")
 
 
 class _Mesh(C.Structure):

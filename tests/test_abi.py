@@ -19,13 +19,14 @@ def _declared(header="start_gpu.h"):
 
 
 def test_every_header_is_checked():
-    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["start_gpu.h", "start_mesh.h"]
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["start_gpu.h", "start_mesh.h", "start_output.h"]
 
 
-def test_mesh_library_exports_every_declared_symbol():
-    lib = C.CDLL(os.path.join(ROOT, "start_amd", "libstart_mesh.so"))
-    names = _declared("start_mesh.h")
-    assert names == ["start_gmsh_free", "start_gmsh_last_error", "start_gmsh_read"]
+def test_io_library_exports_every_declared_symbol():
+    """libstart_io.so (host only): the mesh reader and the output writers."""
+    lib = C.CDLL(os.path.join(ROOT, "start_amd", "libstart_io.so"))
+    names = _declared("start_mesh.h") + _declared("start_output.h")
+    assert "start_gmsh_read" in names and "start_write_movements_json" in names and len(names) == 9
     for name in names:
         assert hasattr(lib, name), name
 

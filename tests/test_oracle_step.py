@@ -140,3 +140,28 @@ def test_degenerate_segment_skips_collision(box, oracle_lib):
     sim2 = _sim(O, om, model="HS", conc=1e45, dt=1e-15)
     sim2.step(moving, 0)
     assert (moving.vx[0], moving.vy[0], moving.vz[0]) != (11.0, 0.0, 0.0)
+
+
+def test_movement_record_mask(box, oracle_lib):
+    """The particles whose start position a step records (pushed and moved,
+    ParticleDynamicsProcessor.cpp:182-184): not the settled, the slow
+    (|v| < 10 m/s) or those outside the expanded bbox; a degenerate segment is
+    still recorded (the record comes before the move check)."""
+    O = oracle_lib
+    mesh, om = box
+    p = np.array([[0.05, 0.05, 0.05], [0.05, 0.05, 0.05], [0.1006, 0.05, 0.05], [0.03, 0.04, 0.0005],
+                  [0.06, 0.02, 0.07]])
+    v = np.array([[11.0, 0.0, 0.0], [9.0, 0.0, 0.0], [100.0, 0.0, 0.0], [0.0, 0.0, -1100.0],
+                  [200.0, 50.0, 0.0]])
+    st = O.State.new(p, v, np.zeros(5, np.int32), np.arange(5, dtype=np.uint64),
+                     om.locate_many(p[:, 0], p[:, 1], p[:, 2]))
+    sim = _sim(O, om)
+    moved = np.full(5, 7, dtype=np.uint8)
+    assert sim.step(st, 0, moved=moved) == 3
+    assert list(moved) == [1, 0, 0, 1, 1]
+    assert st.status[3] == O.SETTLED  # particle 3 settled in step 0 ...
+    assert sim.step(st, 1, moved=moved) == 2
+    assert list(moved) == [1, 0, 0, 0, 1]  # ... and is not recorded again
+    tiny = _one(O, om, (0.05, 0.05, 0.05), (11.0, 0.0, 0.0))
+    m1 = np.zeros(1, dtype=np.uint8)
+    assert _sim(O, om, dt=1e-25).step(tiny, 0, moved=m1) == 1 and m1[0] == 1 and tiny.x[0] == 0.05

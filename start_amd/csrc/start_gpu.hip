@@ -1321,6 +1321,7 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
     (void)hipEventCreate(&e1);
     start_launch_cfg best_cfg = keep;
     double best_ms = -1.0;
+    std::vector<std::pair<double, start_launch_cfg>> timed;
     // trial_steps steps of configuration g from the snapshot; keeps the fastest
     auto trial = [&](const start_launch_cfg& g) -> int {
         int r = snapshot_restore(c, snap);
@@ -1334,6 +1335,7 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         const double per = ms / trial_steps;
+        timed.emplace_back(per, g);
         if (best_ms < 0.0 || per < best_ms) {
             best_ms = per;
             best_cfg = g;
@@ -1357,6 +1359,18 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
         g.walk = 1 - g.walk;
         if (g.walk) g.block = 256;
         rc = trial(g);
+    }
+    if (rc == START_OK && timed.size() > 1) {
+        // one timing of a few ms differs by some % from run to run: the three
+        // fastest are timed twice more and ranked by the better of those two
+        std::sort(timed.begin(), timed.end(),
+                  [](const auto& a, const auto& b) { return a.first < b.first; });
+        std::vector<start_launch_cfg> top;
+        for (size_t i = 0; i < timed.size() && top.size() < 3; ++i) top.push_back(timed[i].second);
+        best_ms = -1.0;
+        timed.clear();
+        for (int rep = 0; rep < 2 && rc == START_OK; ++rep)
+            for (size_t i = 0; i < top.size() && rc == START_OK; ++i) rc = trial(top[i]);
     }
     int rc2 = snapshot_restore(c, snap);
     snapshot_free(snap);

@@ -1066,7 +1066,9 @@ __global__ void prologue_kernel(double* __restrict__ a, double* __restrict__ b, 
 // S.nsteps steps with the state and the owning tet's data in registers.  A
 // particle whose relocation cannot be certified stops and goes to the work
 // list with the step it stopped at.  Called with a block-uniform tile.
-template <typename R, int BS, int HashBits, bool kBatch, bool kWalk>
+// kTrack: movement records on (S.trk_slot set), a kernel variant of its own so
+// that the streaming kernels without them carry none of their code.
+template <typename R, int BS, int HashBits, bool kBatch, bool kWalk, bool kTrack>
 __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R>& P, const StepParams& S,
                                           const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
                                           WorkItem* __restrict__ wl, unsigned long long* __restrict__ cnt,
@@ -1114,17 +1116,17 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
                 if (r.outcome == kFrozenNow) break;
                 ++processed;
                 moved = true;
-                if (S.trk_slot) track_start<R>(M, S, P.pidx[i], t_before, px, py, pz);
+                if (kTrack) track_start<R>(M, S, P.pidx[i], t_before, px, py, pz);
                 if (r.outcome == kSettledNow) {
                     s.f = (s.f | kSettled) & ~kStrict;
                     P.hit[i] = r.hit_tri;
                     atomicAdd(&tri_count[rk->tri_base + r.hit_tri], 1ull);
                     atomicAdd(&cnt[kCntSettled], 1ull);
-                    if (S.settle || S.trk_slot) {
+                    if (S.settle || kTrack) {
                         double a[3];
                         abs_pos(M, t_before, px, py, pz, a);
                         if (S.settle) record_settle(S.settle, P.pidx[i], step0 + (uint32_t)ks, a[0], a[1], a[2]);
-                        if (S.trk_slot) track_hit<R>(M, S, P.pidx[i], a, s.t, s.x, s.y, s.z, r.hit_tri);
+                        if (kTrack) track_hit<R>(M, S, P.pidx[i], a, s.t, s.x, s.y, s.z, r.hit_tri);
                     }
                     break;
                 }
@@ -1192,7 +1194,7 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
 // a grid sized to the chip pulls tiles from a queue counter until the live
 // prefix is covered (one launch for every run of a batch, whatever their
 // sizes).  The LDS hash has 8 slots per thread (12 B each: 24 KB at BS = 256).
-template <typename R, int BS, bool kBatch, bool kPersistent, bool kWalk>
+template <typename R, int BS, bool kBatch, bool kPersistent, bool kWalk, bool kTrack = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) step_main(DevMesh M, DevParticles<R> P, StepParams S,
                                                        const SpeciesK* __restrict__ sp,
                                                        double* __restrict__ node_charge, WorkItem* __restrict__ wl,
@@ -1221,7 +1223,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) st
     const int64_t n_active = *S.n_active;
     if (!kPersistent) {
         if ((int64_t)blockIdx.x * BS < n_active)  // else the whole block is past the live particles
-            step_tile<R, BS, kBits, kBatch, kWalk>(M, P, S, sp, node_charge, wl, cnt, tri_count, blockIdx.x, n_active, step0,
+            step_tile<R, BS, kBits, kBatch, kWalk, kTrack>(M, P, S, sp, node_charge, wl, cnt, tri_count, blockIdx.x, n_active, step0,
                                             &blk_processed, hk, hv);
         return;
     }
@@ -1230,7 +1232,7 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) st
         __syncthreads();
         const int64_t tile = s_tile;
         if (tile * BS >= n_active) break;  // queue drained: every workgroup exits here
-        step_tile<R, BS, kBits, kBatch, kWalk>(M, P, S, sp, node_charge, wl, cnt, tri_count, tile, n_active, step0,
+        step_tile<R, BS, kBits, kBatch, kWalk, kTrack>(M, P, S, sp, node_charge, wl, cnt, tri_count, tile, n_active, step0,
                                         &blk_processed, hk, hv);
     }
 }
@@ -1651,7 +1653,19 @@ void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, cons
                  const SpeciesK* sp, double* node_charge, WorkItem* wl, unsigned long long* cnt,
                  unsigned long long* tri_count, const StepLaunch& L, hipEvent_t ev_mid)
 {
-    if (P.n > 0) {
+    if (P.n > 0 && S.trk_slot) {
+        // movement records: the tracking variants (256-thread workgroups, one per tile)
+        const unsigned grid = blocks_for(P.n, 256);
+        switch ((L.batch ? 2 : 0) + (L.walk ? 1 : 0)) {
+#define SG_STEP_TRACK(B_, W_) \
+    step_main<R, 256, B_, false, W_, true><<<grid, 256, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count)
+        case 0: SG_STEP_TRACK(false, false); break;
+        case 1: SG_STEP_TRACK(false, true); break;
+        case 2: SG_STEP_TRACK(true, false); break;
+        default: SG_STEP_TRACK(true, true); break;
+#undef SG_STEP_TRACK
+        }
+    } else if (P.n > 0) {
         const unsigned grid = L.persistent ? (unsigned)L.grid : blocks_for(P.n, L.block == 128 ? 128 : 256);
         // the multi-tet walk is instantiated for 256-thread workgroups only
         // (the host sets block = 256 with it)

@@ -353,15 +353,35 @@ __device__ inline void lds_charge_add(int32_t* hk, double* hv, double* __restric
     global_add_f64(&node_charge[node], val);  // crowded table: straight to memory
 }
 
-// base: offset of the run's node-charge array (0 for a single simulation)
-template <int Bits>
-__device__ inline void lds_scatter4(int32_t* hk, double* hv, double* __restrict__ node_charge, const int4& nv,
-                                    int32_t base, double q, const double lam[4])
+// Node window: a block's tet-sorted particles touch nodes within a few
+// thousand ids of the lowest one (the node numbering of a structured mesh
+// puts a cell's nodes one layer, (nx+1)(ny+1) ids, apart), so nodes in
+// [wbase, wbase + Win) are summed in a plain LDS array by an LDS FP64 add,
+// without the hash's compare-and-swap; the others go to the hash.
+template <int Bits, int Win>
+__device__ inline void lds_window_add(int32_t* hk, double* hv, double* wv, int32_t wbase,
+                                      double* __restrict__ node_charge, int32_t node, double val)
 {
-    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.x, q * lam[0]);
-    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.y, q * lam[1]);
-    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.z, q * lam[2]);
-    lds_charge_add<Bits>(hk, hv, node_charge, base + nv.w, q * lam[3]);
+    if (Win > 0) {
+        const uint32_t off = (uint32_t)(node - wbase);
+        if (off < (uint32_t)Win) {
+            lds_add_f64(&wv[off], val);
+            return;
+        }
+    }
+    lds_charge_add<Bits>(hk, hv, node_charge, node, val);
+}
+
+// base: offset of the run's node-charge array (0 for a single simulation)
+template <int Bits, int Win>
+__device__ inline void lds_scatter4(int32_t* hk, double* hv, double* wv, int32_t wbase,
+                                    double* __restrict__ node_charge, const int4& nv, int32_t base, double q,
+                                    const double lam[4])
+{
+    lds_window_add<Bits, Win>(hk, hv, wv, wbase, node_charge, base + nv.x, q * lam[0]);
+    lds_window_add<Bits, Win>(hk, hv, wv, wbase, node_charge, base + nv.y, q * lam[1]);
+    lds_window_add<Bits, Win>(hk, hv, wv, wbase, node_charge, base + nv.z, q * lam[2]);
+    lds_window_add<Bits, Win>(hk, hv, wv, wbase, node_charge, base + nv.w, q * lam[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -505,14 +525,16 @@ __device__ inline void to_frame(const DevMesh& M, int32_t t, const double* p, fl
 // ---------------------------------------------------------------------------
 // charge sinks: LDS hash (streaming kernel) or direct atomics (rare paths)
 // ---------------------------------------------------------------------------
-template <int Bits>
+template <int Bits, int Win>
 struct LdsSink {
     int32_t* hk;
     double* hv;
+    double* wv;
+    int32_t wbase;
     double* nc;
     __device__ void add4(const int4& nv, int32_t base, double q, const double lam[4])
     {
-        lds_scatter4<Bits>(hk, hv, nc, nv, base, q, lam);
+        lds_scatter4<Bits, Win>(hk, hv, wv, wbase, nc, nv, base, q, lam);
     }
 };
 struct GlobalSink {
@@ -578,6 +600,48 @@ struct RegSink {
         }
         if (!head) cur = -1;  // summed into the run's first lane
         flush(M);
+    }
+    // flush_wave with the node window of an LDS sink, when no charge has gone
+    // to LDS yet in this tile (single-step launches): the window starts at the
+    // lowest node the tile's runs touch (a block minimum, s_wbase preset to
+    // INT32_MAX); called by every thread of the workgroup.
+    __device__ void flush_wave_window(const DevMesh& M, int32_t* s_wbase)
+    {
+        const int lane = threadIdx.x & 63;
+        const int32_t up_t = __shfl_up(cur, 1), up_b = __shfl_up(base, 1);
+        const bool head = lane == 0 || up_t != cur || up_b != base;
+        const unsigned long long hm = __ballot(head);
+        const int seg = __popcll(hm & (~0ull >> (63 - lane)));
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int sd = __shfl_down(seg, d);
+            const double b0 = __shfl_down(a0, d), b1 = __shfl_down(a1, d), b2 = __shfl_down(a2, d),
+                         b3 = __shfl_down(a3, d);
+            if (lane + d < 64 && sd == seg) {
+                a0 += b0;
+                a1 += b1;
+                a2 += b2;
+                a3 += b3;
+            }
+        }
+        if (!head) cur = -1;
+        int4 nv = make_int4(0, 0, 0, 0);
+        int32_t lo = INT32_MAX;
+        if (cur >= 0) {
+            nv = M.tnode[cur];
+            lo = base + min(min(nv.x, nv.y), min(nv.z, nv.w));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) lo = min(lo, __shfl_xor(lo, off));
+        if (lane == 0) atomicMin(s_wbase, lo);
+        __syncthreads();
+        inner.wbase = __builtin_amdgcn_readfirstlane(*s_wbase);
+        if (cur >= 0) {
+            double v[4] = {a0, a1, a2, a3};
+            inner.add4(nv, base, 1.0, v);
+        }
+        cur = -1;
+        a0 = a1 = a2 = a3 = 0.0;
     }
 };
 
@@ -1068,24 +1132,36 @@ __global__ void prologue_kernel(double* __restrict__ a, double* __restrict__ b, 
 // list with the step it stopped at.  Called with a block-uniform tile.
 // kTrack: movement records on (S.trk_slot set), a kernel variant of its own so
 // that the streaming kernels without them carry none of their code.
-template <typename R, int BS, int HashBits, bool kBatch, bool kWalk, bool kTrack>
+template <typename R, int BS, int HashBits, int Win, bool kBatch, bool kWalk, bool kTrack>
 __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R>& P, const StepParams& S,
                                           const SpeciesK* __restrict__ sp, double* __restrict__ node_charge,
                                           WorkItem* __restrict__ wl, unsigned long long* __restrict__ cnt,
                                           unsigned long long* __restrict__ tri_count, int64_t tile, int64_t n_active,
-                                          uint32_t step0, unsigned int* blk_processed, int32_t* hk, double* hv)
+                                          uint32_t step0, unsigned int* blk_processed, int32_t* hk, double* hv,
+                                          double* wv, int32_t* s_wbase)
 {
+    // The node window is used by single-step launches, whose charge all goes
+    // to LDS in the flush at the end of the tile (flush_wave_window); fused
+    // launches add to LDS while they step and use the hash alone.  (Laying a
+    // hash of twice the slots over the unused window for them was no faster.)
     constexpr int kSlots = 1 << HashBits;
-    if (threadIdx.x == 0) *blk_processed = 0u;
+    const bool win = Win > 0 && S.nsteps == 1;
+    if (threadIdx.x == 0) {
+        *blk_processed = 0u;
+        if (Win > 0) *s_wbase = INT32_MAX;
+    }
     if (S.charge) {
         for (int h = threadIdx.x; h < kSlots; h += BS) {
             hk[h] = -1;
             hv[h] = 0.0;
         }
+        if (win)
+            for (int h = threadIdx.x; h < Win; h += BS) wv[h] = 0.0;
     }
     __syncthreads();
-    RegSink<LdsSink<HashBits>> sink{LdsSink<HashBits>{hk, hv, node_charge}, -1, 0, 0.0, 0.0, 0.0, 0.0};
     int64_t i = tile * BS + threadIdx.x;
+    RegSink<LdsSink<HashBits, Win>> sink{LdsSink<HashBits, Win>{hk, hv, wv, INT32_MAX, node_charge}, -1, 0, 0.0,
+                                         0.0, 0.0, 0.0};
     int processed = 0;
     bool deferred = false;
     int defer_k = 0;
@@ -1150,7 +1226,10 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
             }
         }
     }
-    if (S.charge) sink.flush_wave(M);  // every lane is active here
+    if (S.charge) {  // every lane is active here
+        if (Win > 0 && S.nsteps == 1) sink.flush_wave_window(M, s_wbase);
+        else sink.flush_wave(M);
+    }
     // work list: one atomic per wavefront, lanes take consecutive slots
     unsigned long long dm = __ballot(deferred);
     if (dm) {
@@ -1186,6 +1265,14 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
             int32_t node = hk[h];
             if (node >= 0) global_add_f64(&node_charge[node], hv[h]);
         }
+        // window slots that received charge (untouched ones hold 0.0)
+        if (win) {
+            const int32_t wbase = sink.inner.wbase;
+            for (int h = threadIdx.x; h < Win; h += BS) {
+                const double v = wv[h];
+                if (v != 0.0) global_add_f64(&node_charge[wbase + h], v);
+            }
+        }
     }
     __syncthreads();  // the LDS table and counter are reused by the next tile
 }
@@ -1193,20 +1280,31 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
 // Streaming kernel.  With persistent == 0 workgroup b takes tile b; otherwise
 // a grid sized to the chip pulls tiles from a queue counter until the live
 // prefix is covered (one launch for every run of a batch, whatever their
-// sizes).  The LDS hash has 8 slots per thread (12 B each: 24 KB at BS = 256).
-template <typename R, int BS, bool kBatch, bool kPersistent, bool kWalk, bool kTrack = false>
+// sizes).  The charge of a tile is summed in LDS (node window and hash) and
+// flushed with one FP64 atomic per distinct node.
+template <typename R, int BS, bool kBatch, bool kPersistent, bool kWalk, bool kTrack = false, bool kWindow = false>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) step_main(DevMesh M, DevParticles<R> P, StepParams S,
                                                        const SpeciesK* __restrict__ sp,
                                                        double* __restrict__ node_charge, WorkItem* __restrict__ wl,
                                                        unsigned long long* __restrict__ cnt,
                                                        unsigned long long* __restrict__ tri_count)
 {
-    constexpr int kBits = BS == 256 ? 11 : 10;
-    static_assert((1 << kBits) == 8 * BS, "8 hash slots per thread");
+    // kWindow (single-step launches of one simulation, 256 threads): a
+    // 3,072-node window (24 KB) and a 1,024-slot hash (12 KB), 36 KB in all,
+    // so that four workgroups (the 16 waves the 128-VGPR budget allows) share
+    // a CU's 160 KB.  Otherwise the hash alone, 8 slots per thread: fused
+    // launches add to LDS while they step, and batches interleave their runs
+    // in the tet order.  (The window variant run fused was 5% slower on the
+    // 3M-tet chamber, whose node layers lie 6,600 ids apart.)
+    static_assert(!kWindow || (BS == 256 && !kBatch), "the window is for 256-thread single-run launches");
+    constexpr int kWin = kWindow ? 3072 : 0;
+    constexpr int kBits = kWin > 0 ? 10 : (BS == 256 ? 11 : 10);
     __shared__ unsigned int blk_processed;
     __shared__ int64_t s_tile;
+    __shared__ int32_t s_wbase;
     __shared__ int32_t hk[1 << kBits];
     __shared__ double hv[1 << kBits];
+    __shared__ double wv[kWin > 0 ? kWin : 1];
     const uint32_t step0 = (uint32_t)cnt[kCntStep + S.work_slot];
     {
         // clear the next launch's charge buffers and counters, and give it its first step
@@ -1223,8 +1321,9 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) st
     const int64_t n_active = *S.n_active;
     if (!kPersistent) {
         if ((int64_t)blockIdx.x * BS < n_active)  // else the whole block is past the live particles
-            step_tile<R, BS, kBits, kBatch, kWalk, kTrack>(M, P, S, sp, node_charge, wl, cnt, tri_count, blockIdx.x, n_active, step0,
-                                            &blk_processed, hk, hv);
+            step_tile<R, BS, kBits, kWin, kBatch, kWalk, kTrack>(M, P, S, sp, node_charge, wl, cnt, tri_count,
+                                                                 blockIdx.x, n_active, step0, &blk_processed, hk, hv,
+                                                                 wv, &s_wbase);
         return;
     }
     for (;;) {
@@ -1232,8 +1331,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) st
         __syncthreads();
         const int64_t tile = s_tile;
         if (tile * BS >= n_active) break;  // queue drained: every workgroup exits here
-        step_tile<R, BS, kBits, kBatch, kWalk, kTrack>(M, P, S, sp, node_charge, wl, cnt, tri_count, tile, n_active, step0,
-                                        &blk_processed, hk, hv);
+        step_tile<R, BS, kBits, kWin, kBatch, kWalk, kTrack>(M, P, S, sp, node_charge, wl, cnt, tri_count, tile,
+                                                             n_active, step0, &blk_processed, hk, hv, wv, &s_wbase);
     }
 }
 
@@ -1665,6 +1764,15 @@ void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, cons
         default: SG_STEP_TRACK(true, true); break;
 #undef SG_STEP_TRACK
         }
+    } else if (P.n > 0 && S.nsteps == 1 && !L.persistent && !L.batch && (L.walk || L.block != 128)) {
+        // single-step launches of one simulation: the node-window variants
+        const unsigned grid = blocks_for(P.n, 256);
+        if (L.walk)
+            step_main<R, 256, false, false, true, false, true>
+                <<<grid, 256, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
+        else
+            step_main<R, 256, false, false, false, false, true>
+                <<<grid, 256, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
     } else if (P.n > 0) {
         const unsigned grid = L.persistent ? (unsigned)L.grid : blocks_for(P.n, L.block == 128 ? 128 : 256);
         // the multi-tet walk is instantiated for 256-thread workgroups only

@@ -1798,9 +1798,11 @@ void launch_step(hipStream_t s, const DevMesh& M, const DevParticles<R>& P, cons
         }
     }
     if (ev_mid) (void)hipEventRecord(ev_mid, s);
-    // the work list is only known on the device: a grid of up to 4096
-    // single-wave workgroups, the rest of the list by grid stride
-    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 4096u));
+    // the work list is only known on the device: a grid of up to 1,024
+    // single-wave workgroups -- one per SIMD, all that can be resident at the
+    // kernel's 300+ VGPRs -- and the rest of the list by grid stride.  (4,096
+    // cost 10 us per launch with an empty list: 13% of a single step.)
+    unsigned nb = std::max(1u, std::min(blocks_for(P.n, 64), 1024u));
     if (L.walk)
         step_resolve<R, true><<<nb, 64, 0, s>>>(M, P, S, sp, node_charge, wl, cnt, tri_count);
     else

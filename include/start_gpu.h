@@ -248,9 +248,14 @@ int start_autotune(start_ctx* ctx, int32_t trial_steps, start_launch_cfg* best, 
  * tracked particles; otherwise (*n in = capacity) ids[s], counts[s] (points
  * recorded, possibly above max_points) and xyz[3 * (s * max_points + k)] for
  * slot s.  The JSON of saveMovementsToJson is written on the host from these
- * (start_write_movements_json, include/start_output.h). */
+ * (start_write_movements_json, include/start_output.h).
+ * start_clear_movements empties every slot and keeps the tracked particles:
+ * a caller that appends each call's points to its own map (the
+ * particleMovementMap of process) reads and clears after every step, with
+ * max_points = 2 (a start and a hit point per step). */
 int start_track_movements(start_ctx* ctx, int64_t max_particles, int32_t max_points);
 int start_read_movements(start_ctx* ctx, int64_t* n, uint64_t* ids, int32_t* counts, double* xyz);
+int start_clear_movements(start_ctx* ctx);
 
 /* ---- adjoint of the collisionless push and the surface-hit map ----
  * (no counterpart in the reference; gives d(deposition moments)/d(electrode

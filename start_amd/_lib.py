@@ -28,6 +28,7 @@ EXPORTS = [
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
     "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
     "start_read_tet_field", "start_generate", "start_track_movements", "start_read_movements",
+    "start_clear_movements",
 ]
 SOURCE = {"point": 0, "surface": 1}
 ENERGY = {"mono": 0, "maxwellian": 1, "thompson": 2, "lognormal": 3}
@@ -136,6 +137,7 @@ def lib() -> C.CDLL:
     L.start_adjoint_push.argtypes = [vp, P(d), i32, P(d)]
     L.start_track_movements.argtypes = [vp, i64, i32]
     L.start_read_movements.argtypes = [vp, P(i64), P(u64), P(i32), P(d)]
+    L.start_clear_movements.argtypes = [vp]
     for name in EXPORTS:
         f = getattr(L, name)
         if f.restype is C.c_int:  # default

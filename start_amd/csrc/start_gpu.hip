@@ -1459,6 +1459,15 @@ int start_read_movements(start_ctx* c, int64_t* n, uint64_t* ids, int32_t* count
     return check_kernel_errors(c);
 }
 
+int start_clear_movements(start_ctx* c)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    const int64_t m = (int64_t)c->trk_rows.size();
+    if (m == 0) return START_OK;
+    HIPCHECK(c, hipMemsetAsync(c->d_trk_count, 0, sizeof(int32_t) * (size_t)m, c->stream));
+    return START_OK;
+}
+
 int start_record_settle(start_ctx* c, int32_t on)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");

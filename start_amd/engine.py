@@ -226,6 +226,10 @@ class ParticleInCellStep:
         xyz = xyz[: m * kmax * 3].reshape(m, kmax, 3)
         return {int(ids[s]): xyz[s, : min(int(counts[s]), kmax)].copy() for s in range(m)}
 
+    def clear_movements(self):
+        """Empties the recorded points; the tracked particles stay."""
+        self._check(_lib.lib().start_clear_movements(self.ctx))
+
     def movement_counts(self) -> tuple[np.ndarray, np.ndarray]:
         """(ids, points recorded) per tracked particle, counts possibly above max_points."""
         L = _lib.lib()

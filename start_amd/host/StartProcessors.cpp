@@ -201,6 +201,38 @@ void ParticleKernelDevice::ensureParticles(const ParticleVector& particles)
     m_n = n;
     m_settled_seen = 0;
     m_charge_fresh = false;
+    m_tracking = false;  // a new population ends the device's records
+}
+
+void ParticleKernelDevice::startMovements()
+{
+    if (m_tracking || m_opt.movementMaxParticles == 0 || m_n == 0) return;
+    // at most a start and a hit point per step, read after every step
+    check(start_track_movements(m_ctx, (int64_t)m_opt.movementMaxParticles, 2), "movement tracking");
+    m_tracking = true;
+}
+
+void ParticleKernelDevice::appendMovements(ParticleMovementMap& map, std::mutex& mutex)
+{
+    if (!m_tracking) return;
+    int64_t n = 0;
+    check(start_read_movements(m_ctx, &n, nullptr, nullptr, nullptr), "movement count");
+    if (n == 0) return;
+    std::vector<uint64_t> ids((size_t)n);
+    std::vector<int32_t> counts((size_t)n);
+    std::vector<double> xyz((size_t)n * 2 * 3);
+    check(start_read_movements(m_ctx, &n, ids.data(), counts.data(), xyz.data()), "movements");
+    check(start_clear_movements(m_ctx), "movement reset");
+    std::lock_guard<std::mutex> lock(mutex);  // ParticleMovementTracker.cpp:10
+    for (int64_t s = 0; s < n; ++s) {
+        const int32_t k = std::min<int32_t>(counts[(size_t)s], 2);
+        if (k == 0) continue;
+        auto& pts = map[(size_t)ids[(size_t)s]];
+        for (int32_t j = 0; j < k; ++j) {
+            const double* q = &xyz[((size_t)s * 2 + (size_t)j) * 3];
+            pts.push_back({q[0], q[1], q[2]});
+        }
+    }
 }
 
 void ParticleKernelDevice::step(ParticleVector& particles, ParticlesIDSet& settled, StopSubject& stop)
@@ -294,8 +326,8 @@ void ParticleDynamicsProcessor::process(std::string_view /*config_filename*/, Pa
                                         ParticleKernelDevice& device, ParticleTrackerMap& /*particleTracker*/,
                                         ParticlesIDSet& settledParticlesIds,
                                         std::shared_mutex& /*sh_mutex_settledParticlesCounterMap*/,
-                                        std::mutex& /*mutex_particlesMovementMap*/,
-                                        ParticleMovementMap& /*particleMovementMap*/, StopSubject& stopSubject,
+                                        std::mutex& mutex_particlesMovementMap,
+                                        ParticleMovementMap& particleMovementMap, StopSubject& stopSubject,
                                         std::string_view scatteringModel, std::string_view gasName,
                                         double gasConcentration)
 {
@@ -303,7 +335,9 @@ void ParticleDynamicsProcessor::process(std::string_view /*config_filename*/, Pa
         device.ensureContext();
         device.configure(scatteringModel, gasName, gasConcentration, timeStep);
         device.ensureParticles(particles);
+        device.startMovements();
         device.step(particles, settledParticlesIds, stopSubject);
+        device.appendMovements(particleMovementMap, mutex_particlesMovementMap);
     } catch (std::exception const& ex) {
         std::cerr << "ERROR: particle step on the GPU failed: " << ex.what() << std::endl;
     } catch (...) {

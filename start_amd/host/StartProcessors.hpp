@@ -92,6 +92,9 @@ struct DeviceKernelOptions {
     uint32_t run = 0;
     double bboxExpansion = 1.01;       // Constants.hpp:55
     bool writeBackEachStep = true;     // update the ParticleVector after every process()
+    // particleMovementMap of process(): the first movementMaxParticles moving
+    // particles are recorded (ParticleMovementTracker.hpp:43); 0 records none
+    size_t movementMaxParticles = 200000;
     // Collision diameters r_p + r_t by particle type; types not listed use the
     // reference's radii sum (HSModel.cpp:8-12, VHSModel.cpp:14).
     std::map<ParticleType, double> diameterOverride;
@@ -142,6 +145,10 @@ private:
     void check(int rc, const char* what);
     void step(ParticleVector& particles, ParticlesIDSet& settled, StopSubject& stop);
     void densities(NodeChargeDensitiesMap& out);
+    // movement records: started before the first step of a population, then
+    // each step's points appended to the caller's map
+    void startMovements();
+    void appendMovements(ParticleMovementMap& map, std::mutex& mutex);
 
     DeviceMeshInput m_mesh;
     DeviceKernelOptions m_opt;
@@ -155,14 +162,17 @@ private:
     uint64_t m_settled_seen = 0;
     int64_t m_steps = 0;
     bool m_charge_fresh = false;  // device node charge belongs to the current positions
+    bool m_tracking = false;      // movement records on for the loaded population
 };
 
 class ParticleDynamicsProcessor {
 public:
     // One step of every particle (ParticleDynamicsProcessor.cpp:213-277).  As
     // in the reference it never throws: failures are logged to stderr and the
-    // call returns (:202-209, :268-276).  numThreads, particleTracker, the
-    // two mutexes and particleMovementMap are accepted for the signature; the
+    // call returns (:202-209, :268-276).  particleMovementMap receives the
+    // step's movement records (the start of the step and any hit point) of the
+    // tracked particles, under mutex_particlesMovementMap.  numThreads,
+    // particleTracker and the other mutex are accepted for the signature; the
     // device keeps the owners itself and runs race-free.
     static void process(std::string_view config_filename, ParticleVector& particles, double timeMoment,
                         double timeStep, unsigned int numThreads, ParticleKernelDevice& device,

@@ -123,6 +123,17 @@ int main(int argc, char** argv)
         write_array(dir + "/out_pos.f64", op);
         write_array(dir + "/out_vel.f64", ov);
         write_array(dir + "/out_settled.u64", std::vector<uint64_t>(settled.begin(), settled.end()));
+        std::vector<uint64_t> mv_ids;
+        std::vector<int32_t> mv_counts;
+        std::vector<double> mv_xyz;
+        for (const auto& [id, pts] : movements) {
+            mv_ids.push_back(id);
+            mv_counts.push_back((int32_t)pts.size());
+            for (const auto& p : pts) mv_xyz.insert(mv_xyz.end(), p.begin(), p.end());
+        }
+        write_array(dir + "/out_mv_ids.u64", mv_ids);
+        write_array(dir + "/out_mv_counts.i32", mv_counts);
+        write_array(dir + "/out_mv_xyz.f64", mv_xyz);
         std::vector<uint64_t> counts;
         for (const auto& [tag, c] : device.triangleCounts()) counts.push_back(c);
         write_array(dir + "/out_tri_counts.u64", counts);

@@ -282,9 +282,28 @@ def cpu_baseline(w, budget_s: float = 12.0, max_steps: int = 200):
         processed += sim.step(st, 1 + done, q, nthreads=nth)
         done += 1
     el = time.perf_counter() - t0
+    # the same oracle on one thread, a few more steps of the same state
+    one, t1, k1 = 0, time.perf_counter(), 0
+    while k1 < 5 and time.perf_counter() - t1 < budget_s / 3:
+        q[:] = 0
+        one += sim.step(st, 1 + done + k1, q, nthreads=1)
+        k1 += 1
+    el1 = time.perf_counter() - t1
     return {"value": round(processed / el, 1), "unit": "particle-steps/s", "cores": nth, "kind": "port",
             "sample": f"oracle on the same large_box inputs, steps 1..{done} after 1 warm-up "
-                      f"step ({processed} particle-steps in {el:.1f} s, OpenMP {nth} threads)"}
+                      f"step ({processed} particle-steps in {el:.1f} s, OpenMP {nth} threads)",
+            "single_thread": {"value": round(one / el1, 1), "steps": k1, "particle_steps": one},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def load_traffic(mode: str):

@@ -78,13 +78,14 @@ class BatchPair:
             for key, ref in (("tet", st.tet), ("status", st.status), ("hit", st.hit)):
                 bad = np.nonzero(g[key][sl] != ref)[0]
                 assert len(bad) == 0, f"{where}: run {r} {key} differs at {bad[:5]}"
-            xo = np.stack([st.x, st.y, st.z], 1)
-            xg = np.stack([g["x"][sl], g["y"][sl], g["z"][sl]], 1)
-            vo = np.stack([st.vx, st.vy, st.vz], 1)
-            vg = np.stack([g["vx"][sl], g["vy"][sl], g["vz"][sl]], 1)
-            ex = (np.abs(xo - xg).max(1) / np.linalg.norm(xo, axis=1)).max()
-            ev = (np.abs(vo - vg).max(1) / np.maximum(np.linalg.norm(vo, axis=1), 1e-300)).max()
-            assert ex <= REL_XV and ev <= REL_XV, f"{where}: run {r} x {ex:.2e} v {ev:.2e}"
+            if n:
+                xo = np.stack([st.x, st.y, st.z], 1)
+                xg = np.stack([g["x"][sl], g["y"][sl], g["z"][sl]], 1)
+                vo = np.stack([st.vx, st.vy, st.vz], 1)
+                vg = np.stack([g["vx"][sl], g["vy"][sl], g["vz"][sl]], 1)
+                ex = (np.abs(xo - xg).max(1) / np.linalg.norm(xo, axis=1)).max()
+                ev = (np.abs(vo - vg).max(1) / np.maximum(np.linalg.norm(vo, axis=1), 1e-300)).max()
+                assert ex <= REL_XV and ev <= REL_XV, f"{where}: run {r} x {ex:.2e} v {ev:.2e}"
             assert np.array_equal(tg[r], self.sims[r].tri_count[: self.w.mesh.n_tris]), f"{where}: run {r} counts"
             eq = (np.abs(qg[r] - self.qo[r]) / np.maximum(np.abs(self.qo[r]), E_CHARGE)).max()
             assert eq <= REL_Q, f"{where}: run {r} charge {eq:.2e}"

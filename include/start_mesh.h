@@ -36,6 +36,15 @@ int start_gmsh_read(const char* path, start_gmsh_mesh* out);
 void start_gmsh_free(start_gmsh_mesh* mesh);
 const char* start_gmsh_last_error(void);
 
+/* Writes a mesh as a Gmsh 4.1 ASCII file that Gmsh and the reference's
+ * libgmsh reads (VolumetricMesh.cpp:35-121, GmshUtils.cpp:96-152) open: one
+ * volume entity with every node and the tetrahedra, one surface entity with
+ * the triangles, coordinates printed to 17 significant digits (read back
+ * exactly).  Null tag arrays mean nodes 1..N, triangles 1..S and tets
+ * S+1..S+T.  0 on success; -1 with the reason in start_gmsh_write_error(). */
+int start_gmsh_write(const char* path, const start_gmsh_mesh* mesh);
+const char* start_gmsh_write_error(void);
+
 #ifdef __cplusplus
 }
 #endif

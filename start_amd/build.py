@@ -61,7 +61,8 @@ def build_host(verbose: bool = False) -> list[str]:
         [gxx] + common + ["-o", SHIM_CHECK + ".tmp", os.path.join(HOST, "shim_check.cpp"),
                           "-L", HERE, "-lstart_host", "-lstart_gpu", "-Wl,-rpath,$ORIGIN"],
         [gxx] + common + ["-fPIC", "-shared", "-o", IO_LIB + ".tmp", os.path.join(HOST, "GmshReader.cpp"),
-                          os.path.join(HOST, "Outputs.cpp"), "-Wl,--no-undefined"],
+                          os.path.join(HOST, "GmshWriter.cpp"), os.path.join(HOST, "Outputs.cpp"),
+                          "-Wl,--no-undefined"],
     ]
     outs = [HOST_LIB, SHIM_CHECK, IO_LIB]
     for cmd, out in zip(cmds, outs):

@@ -38,6 +38,9 @@ def _lib():
         L.start_gmsh_free.argtypes = [C.POINTER(_Mesh)]
         L.start_gmsh_free.restype = None
         L.start_gmsh_last_error.restype = C.c_char_p
+        L.start_gmsh_write.argtypes = [C.c_char_p, C.POINTER(_Mesh)]
+        L.start_gmsh_write.restype = C.c_int
+        L.start_gmsh_write_error.restype = C.c_char_p
         _LIB = L
     return _LIB
 
@@ -52,6 +55,23 @@ class GmshMesh:
 
 def _arr(p, n, dtype):
     return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True) if n else np.zeros(0, dtype=dtype)
+
+
+def write(path: str, mesh: TetMesh, node_tags=None, tet_tags=None, tri_tags=None):
+    """The mesh as a Gmsh 4.1 ASCII file (tags default to 1..N, triangles
+    1..S, tets S+1..S+T)."""
+    nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
+    tets = np.ascontiguousarray(mesh.tets, dtype=np.int32)
+    tris = np.ascontiguousarray(mesh.tris, dtype=np.int32)
+    tags = [None if t is None else np.ascontiguousarray(t, dtype=np.uint64) for t in (node_tags, tet_tags, tri_tags)]
+    up = C.POINTER(C.c_uint64)
+    tp = [C.cast(None, up) if t is None else t.ctypes.data_as(up) for t in tags]
+    m = _Mesh(len(nodes), nodes.ctypes.data_as(C.POINTER(C.c_double)), tp[0],
+              len(tets), tets.ctypes.data_as(C.POINTER(C.c_int32)), tp[1],
+              len(tris), tris.ctypes.data_as(C.POINTER(C.c_int32)), tp[2])
+    L = _lib()
+    if L.start_gmsh_write(os.fsencode(path), C.byref(m)) != 0:
+        raise RuntimeError(L.start_gmsh_write_error().decode())
 
 
 def read(path: str) -> GmshMesh:

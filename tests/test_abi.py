@@ -26,7 +26,7 @@ def test_io_library_exports_every_declared_symbol():
     """libstart_io.so (host only): the mesh reader and the output writers."""
     lib = C.CDLL(os.path.join(ROOT, "start_amd", "libstart_io.so"))
     names = _declared("start_mesh.h") + _declared("start_output.h")
-    assert "start_gmsh_read" in names and "start_write_movements_json" in names and len(names) == 9
+    assert "start_gmsh_read" in names and "start_write_movements_json" in names and len(names) == 11
     for name in names:
         assert hasattr(lib, name), name
 

@@ -1349,7 +1349,9 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
         for (int32_t bs : blocks)
             for (int32_t pers = 0; pers <= 1 && rc == START_OK; ++pers)
                 for (int32_t sm = 1; sm <= 4 && rc == START_OK; sm *= 2) {
-                    start_launch_cfg g{K, bs, pers, K * sm, keep.graph, keep.walk};
+                    // timed eagerly: a trial under capture would pay its own
+                    // graph instantiation; the kept shape gets the caller's graph setting
+                    start_launch_cfg g{K, bs, pers, K * sm, 0, keep.walk};
                     if (g.sort_interval > trial_steps) continue;
                     rc = trial(g);
                 }
@@ -1379,6 +1381,7 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
     (void)hipEventDestroy(e1);
     c->timing = timing;
     c->d_trk_slot = trk_keep;
+    best_cfg.graph = keep.graph;
     c->cfg = rc == START_OK ? best_cfg : keep;
     c->prm.steps_per_launch = c->cfg.steps_per_launch;
     c->prm.sort_interval = c->cfg.sort_interval;

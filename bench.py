@@ -36,9 +36,9 @@ BYTES_PER_PS = {"fp64": 296, "mixed": 188}
 
 MODES = {
     # name: (precision, steps fused per launch, sort interval)
-    "fp64_fused": ("fp64", 100, 200),
+    "fp64_fused": ("fp64", 200, 200),
     "fp64_per_step": ("fp64", 1, 50),
-    "mixed_fused": ("mixed", 100, 200),
+    "mixed_fused": ("mixed", 200, 200),
     "mixed_per_step": ("mixed", 1, 50),
 }
 HEADLINE = "fp64_fused"

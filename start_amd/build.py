@@ -27,18 +27,39 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+OBJ = os.path.join(HERE, "_build")
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-munsafe-fp-atomics",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
 def build_library(verbose: bool = False) -> str:
-    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", "-munsafe-fp-atomics",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-result", "-Wl,--no-undefined", "-I", os.path.join(ROOT, "include"),
-           "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+    """Every source compiled to an object in parallel (the step kernels are the
+    long pole), then one link."""
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = []
+    for s in SOURCES:
+        obj = os.path.join(OBJ, os.path.splitext(s)[0] + ".o")
+        cmd = [hipcc()] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", "-o", obj,
+                                       os.path.join(CSRC, s)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((s, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for s, obj, p in jobs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(s)
+    if failed:
+        raise RuntimeError("hipcc failed on " + ", ".join(failed))
+    cmd = [hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared", "-Wl,--no-undefined", "-o", LIB + ".tmp"] + \
+        [obj for _, obj, _ in jobs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libstart_gpu.so")
+        raise RuntimeError("hipcc failed linking libstart_gpu.so")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -339,6 +339,17 @@ int start_synchronize(start_ctx* ctx);
 /* Device point location of n points (owner rule of the step), for checks. */
 int start_locate(start_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                  int32_t* out_tet);
+/* start_locate without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (allocation-free
+ * injection or serving loops that locate a bounded batch of points each step,
+ * harnesses that forbid hipMalloc).  The points and owners are staged in one
+ * static device array per GPU of START_NOALLOC_LOCATE_MAX points (3.5 MiB),
+ * held until the process ends; n above that returns START_ERR_ARG.  Calls
+ * on one GPU share that array: calls that overlap on one GPU (other threads or
+ * contexts on the same device) race. */
+#define START_NOALLOC_LOCATE_MAX 131072
+int start_locate_noalloc(start_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
+                         int32_t* out_tet);
 /* Device exact orientation signs of n quadruples (12 doubles each), for checks. */
 int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign);
 

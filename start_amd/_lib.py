@@ -28,8 +28,9 @@ EXPORTS = [
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
     "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
     "start_read_tet_field", "start_generate", "start_track_movements", "start_read_movements",
-    "start_clear_movements",
+    "start_clear_movements", "start_locate_noalloc",
 ]
+NOALLOC_LOCATE_MAX = 131072
 SOURCE = {"point": 0, "surface": 1}
 ENERGY = {"mono": 0, "maxwellian": 1, "thompson": 2, "lognormal": 3}
 ANGLE = {"fan": 0, "cosine": 1, "isotropic": 2, "cone": 3}
@@ -116,6 +117,7 @@ def lib() -> C.CDLL:
     L.start_read_stats.argtypes = [vp, P(Stats)]
     L.start_synchronize.argtypes = [vp]
     L.start_locate.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
+    L.start_locate_noalloc.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
     L.start_orient3d_batch.argtypes = [i64, P(d), P(i32)]
     L.start_timing_enable.argtypes = [vp, i32]
     L.start_timing_read.argtypes = [vp, P(d), P(u64)]

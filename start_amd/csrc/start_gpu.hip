@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -57,6 +58,13 @@ hipError_t sort_particles(hipStream_t s, const DevParticles<R>& P, R* x, R* y, R
 
 using namespace sg;
 
+// Staging of start_locate_noalloc: the x, y and z columns (FP64) and the
+// owners (int32) of at most START_NOALLOC_LOCATE_MAX points, in one file-scope
+// device array.  Every GPU the library runs on holds its own copy until the
+// process ends; every call on a GPU shares that GPU's copy.
+constexpr size_t kNoAllocBytes = (size_t)START_NOALLOC_LOCATE_MAX * (3 * sizeof(double) + sizeof(int32_t));
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace {
 
 std::string g_create_error;
@@ -89,6 +97,31 @@ bool same_cfg(const start_launch_cfg& a, const start_launch_cfg& b)
            a.sort_interval == b.sort_interval && a.walk == b.walk;
 }
 
+// Device address of `scratch` per device.  In host code the array's name is a
+// host-side handle, not a device address, so kernels and copies take these.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address on the current device, resolved once per device (called by
+// start_create with the context's device current).
+hipError_t scratch_address(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct start_ctx {
@@ -98,6 +131,7 @@ struct start_ctx {
     start_params prm{};
     start_launch_cfg cfg{};
     int n_cu = 256;
+    void* scratch = nullptr;  // device address of the no-allocation staging array on this context's device
     std::vector<start_species_desc> species_desc;
     bool mixed = false;
     size_t es = sizeof(double);  // bytes of one position/velocity component
@@ -828,6 +862,7 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         if (he == hipSuccess && x != hipSuccess) he = x;
     };
     chk(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    chk(scratch_address(&c->scratch));
     chk(dalloc(&c->d_nodes, 3 * M));
     chk(dalloc(&c->d_geo, 12 * T));
     chk(dalloc(&c->d_E, 3 * T));
@@ -1772,6 +1807,35 @@ int start_locate(start_ctx* c, int64_t n, const double* x, const double* y, cons
     (void)hipFree(dz);
     (void)hipFree(dt);
     if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("locate: ") + hipGetErrorString(e));
+    return check_kernel_errors(c);
+}
+
+// start_locate with the points and owners staged in the file-scope `scratch`
+// array of this context's device (address resolved by start_create): no device
+// allocation.  Every slot the kernel reads (x, y, z of points 0..n-1) is
+// written by this call first, so nothing needs clearing between calls.
+int start_locate_noalloc(start_ctx* c, int64_t n, const double* x, const double* y, const double* z,
+                         int32_t* out_tet)
+{
+    if (!c || n < 0 || (n > 0 && (!x || !y || !z || !out_tet))) return fail(c, START_ERR_ARG, "bad argument");
+    if (n > START_NOALLOC_LOCATE_MAX)
+        return fail(c, START_ERR_ARG, "start_locate_noalloc takes at most " +
+                                          std::to_string(START_NOALLOC_LOCATE_MAX) + " points per call, got " +
+                                          std::to_string(n));
+    if (n == 0) return START_OK;
+    if (!c->scratch) return fail(c, START_ERR_STATE, "no staging array on this device");
+    double* dx = (double*)c->scratch;
+    double* dy = dx + START_NOALLOC_LOCATE_MAX;
+    double* dz = dy + START_NOALLOC_LOCATE_MAX;
+    int32_t* dt = (int32_t*)(dz + START_NOALLOC_LOCATE_MAX);
+    const size_t b8 = sizeof(double) * (size_t)n;
+    HIPCHECK(c, hipMemcpyAsync(dx, x, b8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dy, y, b8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dz, z, b8, hipMemcpyHostToDevice, c->stream));
+    launch_locate(c->stream, c->mesh_view(), dx, dy, dz, n, dt, nullptr, nullptr, nullptr, nullptr, c->d_cnt);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(out_tet, dt, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     return check_kernel_errors(c);
 }
 

@@ -322,6 +322,14 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_locate(self.ctx, len(pts), _dp(x), _dp(y), _dp(z), _ip(out)))
         return out
 
+    def locate_noalloc(self, pts: np.ndarray) -> np.ndarray:
+        """start_locate without device allocation (at most NOALLOC_LOCATE_MAX points)."""
+        pts = np.asarray(pts, dtype=np.float64)
+        x, y, z = (np.ascontiguousarray(pts[:, k]) for k in range(3))
+        out = np.empty(len(pts), dtype=np.int32)
+        self._check(_lib.lib().start_locate_noalloc(self.ctx, len(pts), _dp(x), _dp(y), _dp(z), _ip(out)))
+        return out
+
     def timing(self, on: bool):
         self._check(_lib.lib().start_timing_enable(self.ctx, int(on)))
 

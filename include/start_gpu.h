@@ -128,6 +128,22 @@ int start_load_particles(start_ctx* ctx, int64_t n, const double* x, const doubl
                          const double* z, const double* vx, const double* vy, const double* vz,
                          const int32_t* species, const uint64_t* ids);
 
+/* Appends m particles after the loaded ones: _spawnParticles inserts newly
+ * generated particles at the end of the ParticleVector every step in
+ * continuous-sputtering mode (ModelingMainDriver.cpp:37-71, :141-142).
+ * Everything already loaded is kept: particle state, settled flags and hit
+ * triangles, triangle counters, node charge, statistics, the step counter, and
+ * the movement and settle records.  The new particles take load indices
+ * n..n+m-1 (their ids default to those indices), so their variate streams are
+ * new and the loaded particles' streams do not change.  They are located like
+ * start_load_particles and join the movement records while fewer particles
+ * than requested are tracked.  The device buffers grow geometrically, so
+ * appending a few particles per step copies the population only now and then.
+ * A single simulation only: START_ERR_STATE for a batch. */
+int start_append_particles(start_ctx* ctx, int64_t m, const double* x, const double* y,
+                           const double* z, const double* vx, const double* vy, const double* vz,
+                           const int32_t* species, const uint64_t* ids);
+
 /* ---- particle sources sampled on the device ----
  * Replaces ParticleGeneratorHost::fromPointSource / fromSurfaceSource
  * (ParticleGeneratorHost.cpp:33-159), their CUDA twins
@@ -350,6 +366,15 @@ int start_locate(start_ctx* ctx, int64_t n, const double* x, const double* y, co
 #define START_NOALLOC_LOCATE_MAX 131072
 int start_locate_noalloc(start_ctx* ctx, int64_t n, const double* x, const double* y, const double* z,
                          int32_t* out_tet);
+/* The barycentric maps and certification thresholds start_create builds, on
+ * the host (no device needed), for checks: tets_out (4 per tet, may be NULL)
+ * the positively oriented connectivity, geo 12 doubles per tet (v0, then the
+ * map A row-major: lambda_{1,2,3} = A (x - v0)), rho per tet (may be NULL) a
+ * bound on the largest row sum of |A E - I| (E the exact edge matrix), tau
+ * the FP64 threshold; geo32 (12 floats per tet: A, then v0) and tau32 the
+ * mixed path's, may be NULL.  An infinite threshold certifies nothing. */
+int start_mesh_maps(const start_mesh_desc* mesh, int32_t* tets_out, double* geo, double* rho,
+                    double* tau, float* geo32, double* tau32);
 /* Device exact orientation signs of n quadruples (12 doubles each), for checks. */
 int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign);
 

@@ -28,7 +28,7 @@ EXPORTS = [
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
     "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
     "start_read_tet_field", "start_generate", "start_track_movements", "start_read_movements",
-    "start_clear_movements", "start_locate_noalloc",
+    "start_clear_movements", "start_locate_noalloc", "start_append_particles", "start_mesh_maps",
 ]
 NOALLOC_LOCATE_MAX = 131072
 SOURCE = {"point": 0, "surface": 1}
@@ -106,6 +106,7 @@ def lib() -> C.CDLL:
     L.start_last_error.argtypes = [vp]
     L.start_last_error.restype = C.c_char_p
     L.start_load_particles.argtypes = [vp, i64] + [P(d)] * 6 + [P(i32), P(u64)]
+    L.start_append_particles.argtypes = [vp, i64] + [P(d)] * 6 + [P(i32), P(u64)]
     L.start_set_tet_field.argtypes = [vp, P(d)]
     L.start_set_node_potential.argtypes = [vp, P(d)]
     L.start_step.argtypes = [vp, i32]
@@ -119,6 +120,7 @@ def lib() -> C.CDLL:
     L.start_locate.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
     L.start_locate_noalloc.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
     L.start_orient3d_batch.argtypes = [i64, P(d), P(i32)]
+    L.start_mesh_maps.argtypes = [P(MeshDesc), P(i32), P(d), P(d), P(d), P(C.c_float), P(d)]
     L.start_timing_enable.argtypes = [vp, i32]
     L.start_timing_read.argtypes = [vp, P(d), P(u64)]
     L.start_node_charge_density.argtypes = [vp, i32, P(d), P(i32)]

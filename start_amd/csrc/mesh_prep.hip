@@ -3,6 +3,8 @@
 
 #include <math.h>
 
+#include <cmath>
+
 #include <algorithm>
 #include <array>
 #include <cstring>
@@ -63,7 +65,135 @@ void build_grid(const HostMesh& m, int64_t nitems, int nv, const std::vector<int
     }
 }
 
+// Error-free transforms (exact on IEEE doubles without contraction):
+// a - b = s + e and a * b = p + e.
+inline void two_diff(double a, double b, double& s, double& e)
+{
+    const double nb = -b;  // Knuth's TwoSum of a and -b
+    s = a + nb;
+    const double bv = s - a;
+    const double av = s - bv;
+    e = (a - av) + (nb - bv);
+}
+inline void two_prod(double a, double b, double& p, double& e)
+{
+    p = a * b;
+    e = std::fma(a, b, -p);
+}
+
+// R = A E - I with E = [v1 - v0, v2 - v0, v3 - v0] exactly (each difference a
+// double-double hi + lo), and a bound err >= |R_computed - R_exact| per entry.
+// A*hi is split exactly; A*lo (|lo| <= eps |hi|) is rounded once; the nine
+// terms of an entry are summed in long double (64-bit significand).
+void map_residual(const double* A, const double hi[3][3], const double lo[3][3], long double R[3][3],
+                  double err[3][3])
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            long double acc = r == c ? -1.0L : 0.0L;
+            double mag = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                double p, pe;
+                two_prod(A[3 * r + k], hi[k][c], p, pe);
+                acc += (long double)p;
+                acc += (long double)pe;
+                acc += (long double)(A[3 * r + k] * lo[k][c]);
+                mag += fabs(A[3 * r + k]) * (fabs(hi[k][c]) + fabs(lo[k][c]));
+            }
+            // 10 long-double additions (2^-64 each, relative to the running
+            // magnitude <= mag + 1), the A*lo rounding (2^-53 * 2^-53 mag) and
+            // the rounding of mag itself: 2^-58 (mag + 1) covers all of it
+            R[r][c] = acc;
+            err[r][c] = 0x1p-58 * (mag + 1.0);
+        }
+}
+
+// The barycentric map of a positively oriented tet with vertices P[0..3]:
+// g = v0 (3) then A (3x3, row-major) with lambda_{1,2,3}(x) = A (x - v0).
+// A starts from the long-double cofactor inverse of the edge matrix E, is
+// refined by one Newton step A <- A - R A with the residual R = A E - I taken
+// nearly exactly, and is rounded to double.  Returns the largest row sum of
+// |A|, the largest vertex coordinate difference (diam) and rho, a rigorous
+// bound on the largest row sum of |A E - I| for the stored double A.
+void map_of(const double* const P[4], double* g, double& rowsum, double& diam, double& rho)
+{
+    double hi[3][3], lo[3][3];
+    long double e[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            two_diff(P[c + 1][r], P[0][r], hi[r][c], lo[r][c]);
+            e[r][c] = (long double)hi[r][c] + (long double)lo[r][c];
+        }
+    diam = 0.0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = a + 1; b < 4; ++b)
+            for (int k = 0; k < 3; ++k) diam = std::max(diam, fabs(P[a][k] - P[b][k]));
+    long double det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                      e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                      e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+    long double inv[3][3];
+    inv[0][0] = (e[1][1] * e[2][2] - e[1][2] * e[2][1]) / det;
+    inv[0][1] = (e[0][2] * e[2][1] - e[0][1] * e[2][2]) / det;
+    inv[0][2] = (e[0][1] * e[1][2] - e[0][2] * e[1][1]) / det;
+    inv[1][0] = (e[1][2] * e[2][0] - e[1][0] * e[2][2]) / det;
+    inv[1][1] = (e[0][0] * e[2][2] - e[0][2] * e[2][0]) / det;
+    inv[1][2] = (e[0][2] * e[1][0] - e[0][0] * e[1][2]) / det;
+    inv[2][0] = (e[1][0] * e[2][1] - e[1][1] * e[2][0]) / det;
+    inv[2][1] = (e[0][1] * e[2][0] - e[0][0] * e[2][1]) / det;
+    inv[2][2] = (e[0][0] * e[1][1] - e[0][1] * e[1][0]) / det;
+    double A[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) A[3 * r + c] = (double)inv[r][c];
+    // one Newton step: the cofactor inverse errs by ~2^-64 kappa relative, more
+    // than the double rounding once kappa > 2^11; afterwards only that rounding
+    long double R[3][3];
+    double err[3][3];
+    map_residual(A, hi, lo, R, err);
+    long double A1[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            long double corr = 0.0L;
+            for (int k = 0; k < 3; ++k) corr += R[r][k] * (long double)A[3 * k + c];
+            A1[3 * r + c] = (long double)A[3 * r + c] - corr;
+        }
+    for (int q = 0; q < 9; ++q) A[q] = (double)A1[q];
+    map_residual(A, hi, lo, R, err);
+    rowsum = 0.0;
+    rho = 0.0;
+    for (int r = 0; r < 3; ++r) {
+        double srow = 0.0, rrow = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            srow += fabs(A[3 * r + c]);
+            rrow += fabsl(R[r][c]) + err[r][c];
+        }
+        rowsum = std::max(rowsum, srow);
+        rho = std::max(rho, rrow * (1.0 + 1e-12));
+    }
+    g[0] = P[0][0];
+    g[1] = P[0][1];
+    g[2] = P[0][2];
+    for (int q = 0; q < 9; ++q) g[3 + q] = A[q];
+}
+
 }  // namespace
+
+// The step kernels evaluate lambda at a point x (double) as
+//   d = x - v0;  l_r = (A_r0 d0 + A_r1 d1) + A_r2 d2;  l_0 = 1 - ((l1 + l2) + l3)
+// (no contraction).  Against the exact mu = E^-1 (x - v0), with
+// |x - v0|_inf <= D and R = A E - I:
+//   |l_r - mu_r| <= (eps + gamma_3) sum_k |A_rk| |x_k - v0_k| + |(R mu)_r|
+//                <= 4.01 eps rowsum D + rho max|mu|,
+//   |l_0 - mu_0| <= sum_r |l_r - mu_r| + gamma_3 (1 + sum |l|).
+// The kernels classify only points with sum |l| <= 2, i.e. within
+// (2 + O(tau)) diam of v0 and with max|mu| <= 2 + O(tau): with D = 2 diam the
+// worst case is 3 (8.02 eps R' + 2 rho) + 9 eps, R' = rowsum diam.  tau =
+// 4 eps (12 R' + 8) + 8 rho is 2x the eps part and 1.33x the rho part.  Both
+// grow linearly in |x - v0|, which the multi-tet walk's scaled tolerance
+// relies on.
+double tau_bound(double rowsum, double diam, double rho)
+{
+    return 4.0 * kEps * (12.0 * rowsum * diam + 8.0) + 8.0 * rho;
+}
 
 std::string prepare_mesh(HostMesh& m, int64_t n_nodes, const double* nodes, int64_t n_tets,
                          const int32_t* tets, int64_t n_tris, const int32_t* tris, double bbox_expansion)
@@ -148,63 +278,23 @@ std::string prepare_mesh(HostMesh& m, int64_t n_nodes, const double* nodes, int6
         if (!found) return "surface triangle " + std::to_string(s) + " is not a face of the tet mesh";
     }
 
-    // 4. barycentric maps (long double inverse of the edge matrix) and the
-    //    certification threshold tau
+    // 4. barycentric maps and the certification threshold tau (see map_of)
     m.geo.assign((size_t)(12 * n_tets), 0.0);
+    m.rho.assign((size_t)n_tets, 0.0);
     double tau = 0.0;
     for (int64_t t = 0; t < n_tets; ++t) {
         const int32_t* v = &m.tets[4 * (size_t)t];
-        const double* p0 = node_ptr(m, v[0]);
-        long double e[3][3];  // columns: v1-v0, v2-v0, v3-v0
-        double diam = 0.0;
-        for (int c = 0; c < 3; ++c) {
-            const double* pc = node_ptr(m, v[c + 1]);
-            for (int r = 0; r < 3; ++r) e[r][c] = (long double)pc[r] - (long double)p0[r];
+        const double* P[4] = {node_ptr(m, v[0]), node_ptr(m, v[1]), node_ptr(m, v[2]), node_ptr(m, v[3])};
+        double rowsum = 0.0, diam = 0.0, rho = 0.0;
+        map_of(P, &m.geo[12 * (size_t)t], rowsum, diam, rho);
+        m.rho[(size_t)t] = rho;
+        if (!(rho < 1e-3)) {  // the inverse is not even approximate: never certify with it
+            tau = INFINITY;
+            continue;
         }
-        for (int a = 0; a < 4; ++a)
-            for (int b = a + 1; b < 4; ++b) {
-                const double* pa = node_ptr(m, v[a]);
-                const double* pb = node_ptr(m, v[b]);
-                for (int k = 0; k < 3; ++k) diam = std::max(diam, fabs(pa[k] - pb[k]));
-            }
-        long double det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
-                          e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
-                          e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
-        long double inv[3][3];
-        inv[0][0] = (e[1][1] * e[2][2] - e[1][2] * e[2][1]) / det;
-        inv[0][1] = (e[0][2] * e[2][1] - e[0][1] * e[2][2]) / det;
-        inv[0][2] = (e[0][1] * e[1][2] - e[0][2] * e[1][1]) / det;
-        inv[1][0] = (e[1][2] * e[2][0] - e[1][0] * e[2][2]) / det;
-        inv[1][1] = (e[0][0] * e[2][2] - e[0][2] * e[2][0]) / det;
-        inv[1][2] = (e[0][2] * e[1][0] - e[0][0] * e[1][2]) / det;
-        inv[2][0] = (e[1][0] * e[2][1] - e[1][1] * e[2][0]) / det;
-        inv[2][1] = (e[0][1] * e[2][0] - e[0][0] * e[2][1]) / det;
-        inv[2][2] = (e[0][0] * e[1][1] - e[0][1] * e[1][0]) / det;
-        double* g = &m.geo[12 * (size_t)t];
-        g[0] = p0[0];
-        g[1] = p0[1];
-        g[2] = p0[2];
-        double rowsum = 0.0;
-        for (int r = 0; r < 3; ++r) {
-            double s = 0.0;
-            for (int c = 0; c < 3; ++c) {
-                g[3 + 3 * r + c] = (double)inv[r][c];
-                s += fabs((double)inv[r][c]);
-            }
-            rowsum = std::max(rowsum, s);
-        }
-        // |computed - exact| of lambda at a point x with |x - v0|_inf <= D, to
-        // first order in eps: map rounding (eps rowsum D) + (x - v0) rounding
-        // (eps rowsum D) + 3-term dot product (3 eps rowsum D) = 5 eps rowsum D
-        // for each of lambda_1..3; lambda_0 = 1 - ((l1 + l2) + l3) adds the
-        // three and 3 eps (1 + sum |lambda_j|) <= 9 eps.  So <= 15 eps R + 9 eps
-        // with R = rowsum D.  The step kernels classify only points with
-        // sum |lambda| <= 2, hence |x - v0|_inf <= 2 diam (+ O(tau)): D = 2 diam,
-        // and tau = 4 eps (6 R + 8) is 1.6x the bound.
-        double bound = 4.0 * kEps * (12.0 * rowsum * diam + 8.0);
-        tau = std::max(tau, bound);
+        tau = std::max(tau, tau_bound(rowsum, diam, rho));
     }
-    if (tau > 1e-6) return "mesh too ill-conditioned for the certified barycentric fast path";
+    if (std::isnan(tau)) return "non-finite barycentric map (degenerate tetrahedron)";
     m.tau = tau;
 
     // 5. bucket grids

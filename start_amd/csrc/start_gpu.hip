@@ -168,8 +168,9 @@ struct start_ctx {
     // node charge density tables (built on first use)
     double *d_tet_vol = nullptr, *d_node_vol = nullptr, *d_tet_q = nullptr;
     int32_t *d_adj_start = nullptr, *d_adj_list = nullptr, *d_tet_n = nullptr;
-    // particles
+    // particles: n live rows in buffers of cap rows (appends grow cap geometrically)
     int64_t n = 0;
+    int64_t cap = 0;
     PBuf A, B;  // live arrays and the sort target
     std::vector<uint64_t> ids;
     WorkItem* d_wl = nullptr;
@@ -190,6 +191,7 @@ struct start_ctx {
     int32_t* d_trk_count = nullptr;
     double* d_trk_xyz = nullptr;
     int32_t trk_max = 0;
+    int64_t trk_cap = 0;  // particles the caller asked to track (appended ones may join)
     std::vector<int64_t> trk_rows;
     // captured step chunks
     std::vector<GraphEntry> graphs;
@@ -309,7 +311,40 @@ void free_tracking(start_ctx* c)
     c->d_trk_slot = c->d_trk_count = nullptr;
     c->d_trk_xyz = nullptr;
     c->trk_max = 0;
+    c->trk_cap = 0;
     c->trk_rows.clear();
+}
+
+// The sort target, work list and sort scratch (contents never outlive a
+// launch or a sort, so a growth reallocates them without copying).
+void free_work_buffers(start_ctx* c)
+{
+    free_pbuf(c->B);
+    void* ptrs[] = {c->d_wl, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    c->d_wl = nullptr;
+    c->d_keys = c->d_keys_out = c->d_idx = c->d_perm = nullptr;
+    c->d_sort_temp = nullptr;
+    c->sort_temp = 0;
+}
+
+hipError_t alloc_work_buffers(start_ctx* c, int64_t cap)
+{
+    hipError_t he = dalloc(&c->d_wl, cap);
+    if (he == hipSuccess) he = alloc_pbuf(c->B, cap, c->es);
+    if (he == hipSuccess) he = dalloc(&c->d_keys, cap);
+    if (he == hipSuccess) he = dalloc(&c->d_keys_out, cap);
+    if (he == hipSuccess) he = dalloc(&c->d_idx, cap);
+    if (he == hipSuccess) he = dalloc(&c->d_perm, cap);
+    if (he == hipSuccess) {
+        int bits = 1;
+        while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
+        c->sort_bits = bits;
+        c->sort_temp = sort_temp_bytes(cap, bits);
+        he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
+    }
+    return he;
 }
 
 void free_particles(start_ctx* c)
@@ -317,16 +352,11 @@ void free_particles(start_ctx* c)
     drop_graphs(c);
     free_tracking(c);
     free_pbuf(c->A);
-    free_pbuf(c->B);
-    void* ptrs[] = {c->d_wl, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp, c->d_settle};
+    free_work_buffers(c);
+    if (c->d_settle) (void)hipFree(c->d_settle);
     c->d_settle = nullptr;
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    c->d_wl = nullptr;
-    c->d_keys = c->d_keys_out = c->d_idx = c->d_perm = nullptr;
-    c->d_sort_temp = nullptr;
-    c->sort_temp = 0;
     c->n = 0;
+    c->cap = 0;
 }
 
 // radix sort by owning tet, settled particles last, then swap the buffer sets
@@ -376,7 +406,10 @@ SpeciesK species_constants(const start_species_desc& s, const start_gas_desc& g)
 // lambda_0 = 1 - ((l1 + l2) + l3) adds the three and gamma_3 (1 + sum|l_j|),
 // so |lambda32 - lambda| <= (12 R + 7) eps32; tau32 takes 1.25x that at
 // D = 2 diam (the step kernels classify only points with sum |lambda| <= 2,
-// which lie within 2 diam of v0).
+// which lie within 2 diam of v0), plus 8 rho for the FP64 map's own error
+// (rho bounds the row sums of |A E - I|).  A tet whose FP32 bound reaches
+// 0.05 makes FP32 certify nothing: every classification then falls back to
+// the FP64 map (correct, slower).
 float build_geo32(const HostMesh& hm, std::vector<float>& g32)
 {
     const double eps32 = 5.9604644775390625e-08;  // 2^-24
@@ -403,9 +436,9 @@ float build_geo32(const HostMesh& hm, std::vector<float>& g32)
             for (int b = a + 1; b < 4; ++b)
                 for (int k = 0; k < 3; ++k)
                     diam = std::max(diam, fabs(hm.nodes[3 * (size_t)v[a] + k] - hm.nodes[3 * (size_t)v[b] + k]));
-        tau = std::max(tau, 1.25 * eps32 * (12.0 * rowsum * 2.0 * diam + 7.0));
+        tau = std::max(tau, 1.25 * eps32 * (12.0 * rowsum * 2.0 * diam + 7.0) + 8.0 * hm.rho[(size_t)t]);
     }
-    return (float)tau;
+    return tau < 0.05 ? (float)tau : INFINITY;
 }
 
 // Tet volumes ((b-a) . ((c-a) x (d-a)))/6 of the positively oriented tets,
@@ -720,25 +753,14 @@ int begin_population(start_ctx* c, const std::vector<RunK>& rk, int64_t n, const
     rc = alloc_run_outputs(c);
     if (rc != START_OK) return rc;
     hipError_t he = alloc_pbuf(c->A, n, c->es);
-    if (he == hipSuccess) he = dalloc(&c->d_wl, n);
-    if (he == hipSuccess) he = alloc_pbuf(c->B, n, c->es);
-    if (he == hipSuccess) he = dalloc(&c->d_keys, n);
-    if (he == hipSuccess) he = dalloc(&c->d_keys_out, n);
-    if (he == hipSuccess) he = dalloc(&c->d_idx, n);
-    if (he == hipSuccess) he = dalloc(&c->d_perm, n);
-    if (he == hipSuccess) {
-        int bits = 1;
-        while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
-        c->sort_bits = bits;
-        c->sort_temp = sort_temp_bytes(n, bits);
-        he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
-    }
+    if (he == hipSuccess) he = alloc_work_buffers(c, n);
     if (he != hipSuccess) {
         free_particles(c);
         return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
                     std::string("particle allocation failed: ") + hipGetErrorString(he));
     }
     c->n = n;
+    c->cap = n;
     c->ids.assign(ids ? ids : (const uint64_t*)nullptr, ids ? ids + n : (const uint64_t*)nullptr);
     if (!ids) {
         c->ids.resize((size_t)n);
@@ -760,6 +782,194 @@ int begin_population(start_ctx* c, const std::vector<RunK>& rk, int64_t n, const
     c->wslot = 0;
     c->step = 0;
     c->last_sort = 0;
+    return START_OK;
+}
+
+// Room for `need` particles: a larger buffer set (1.5x the old capacity, at
+// least need) with the live rows copied over, and the records kept by load
+// index (settle records, movement slots) extended with empty entries.  No
+// particle state, counter, node charge or step changes.
+int grow_particles(start_ctx* c, int64_t need)
+{
+    if (need <= c->cap) return START_OK;
+    int64_t cap = std::max<int64_t>({need, c->cap + c->cap / 2, 1024});
+    cap = std::min<int64_t>(cap, std::max<int64_t>(need, 0x7fffffff));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured launches hold the old buffers
+    PBuf A2;
+    SettleRec* s2 = nullptr;
+    int32_t* slot2 = nullptr;
+    hipError_t he = alloc_pbuf(A2, cap, c->es);
+    if (he == hipSuccess && c->n > 0) he = copy_pbuf(A2, c->A, c->n, c->es, c->stream);
+    if (he == hipSuccess && c->d_settle) {  // step = -1: not settled
+        he = dalloc(&s2, cap);
+        if (he == hipSuccess) he = hipMemsetAsync(s2, 0xff, sizeof(SettleRec) * (size_t)cap, c->stream);
+        if (he == hipSuccess && c->n > 0)
+            he = hipMemcpyAsync(s2, c->d_settle, sizeof(SettleRec) * (size_t)c->n, hipMemcpyDeviceToDevice,
+                                c->stream);
+    }
+    if (he == hipSuccess && c->d_trk_slot) {  // -1: not tracked
+        he = dalloc(&slot2, cap);
+        if (he == hipSuccess) he = hipMemsetAsync(slot2, 0xff, sizeof(int32_t) * (size_t)cap, c->stream);
+        if (he == hipSuccess && c->n > 0)
+            he = hipMemcpyAsync(slot2, c->d_trk_slot, sizeof(int32_t) * (size_t)c->n, hipMemcpyDeviceToDevice,
+                                c->stream);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he != hipSuccess) {
+        free_pbuf(A2);
+        if (s2) (void)hipFree(s2);
+        if (slot2) (void)hipFree(slot2);
+        return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("particle buffers (growing): ") + hipGetErrorString(he));
+    }
+    free_pbuf(c->A);
+    c->A = A2;
+    if (s2) {
+        (void)hipFree(c->d_settle);
+        c->d_settle = s2;
+    }
+    if (slot2) {
+        (void)hipFree(c->d_trk_slot);
+        c->d_trk_slot = slot2;
+    }
+    free_work_buffers(c);
+    he = alloc_work_buffers(c, cap);
+    if (he != hipSuccess) {
+        free_work_buffers(c);  // the state is kept; start_step refuses until buffers exist
+        return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("work buffers (growing): ") + hipGetErrorString(he));
+    }
+    c->cap = cap;
+    return START_OK;
+}
+
+// Rows [first, first + m) of the live buffers from host arrays: the state
+// (positions in the frame of their owner in mixed mode, velocities in FP32),
+// the flags given, hit = -1, load indices first.., and owners by the locate
+// kernel (min-id closed containment).
+int upload_rows(start_ctx* c, int64_t first, int64_t m, const double* x, const double* y, const double* z,
+                const double* vx, const double* vy, const double* vz, const int32_t* flags)
+{
+    if (m == 0) return START_OK;
+    std::vector<uint32_t> pidx((size_t)m);
+    for (int64_t i = 0; i < m; ++i) pidx[(size_t)i] = (uint32_t)(first + i);
+    std::vector<int32_t> hit((size_t)m, -1);
+    HIPCHECK(c, hipMemcpy(c->A.hit + first, hit.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->A.pidx + first, pidx.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->A.flags + first, flags, sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    const size_t b8 = sizeof(double) * (size_t)m;
+    DevMesh M = c->mesh_view();
+    if (!c->mixed) {
+        const double* src[6] = {x, y, z, vx, vy, vz};
+        void* dst[6] = {c->A.x, c->A.y, c->A.z, c->A.vx, c->A.vy, c->A.vz};
+        for (int k = 0; k < 6; ++k) HIPCHECK(c, hipMemcpy((double*)dst[k] + first, src[k], b8, hipMemcpyHostToDevice));
+        launch_locate(c->stream, M, (double*)c->A.x + first, (double*)c->A.y + first, (double*)c->A.z + first, m,
+                      c->A.tet + first, c->A.flags + first, nullptr, nullptr, nullptr, c->d_cnt);
+    } else {
+        // positions go to the frame of their owner; velocities to FP32
+        double *dx = nullptr, *dy = nullptr, *dz = nullptr;
+        hipError_t e = dalloc(&dx, m);
+        if (e == hipSuccess) e = dalloc(&dy, m);
+        if (e == hipSuccess) e = dalloc(&dz, m);
+        if (e == hipSuccess) e = hipMemcpy(dx, x, b8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dy, y, b8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dz, z, b8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_locate(c->stream, M, dx, dy, dz, m, c->A.tet + first, c->A.flags + first, (float*)c->A.x + first,
+                          (float*)c->A.y + first, (float*)c->A.z + first, c->d_cnt);
+            e = hipStreamSynchronize(c->stream);
+        }
+        if (dx) (void)hipFree(dx);
+        if (dy) (void)hipFree(dy);
+        if (dz) (void)hipFree(dz);
+        if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("particle upload: ") + hipGetErrorString(e));
+        std::vector<float> f((size_t)m);
+        const double* vs[3] = {vx, vy, vz};
+        void* vd[3] = {c->A.vx, c->A.vy, c->A.vz};
+        for (int k = 0; k < 3; ++k) {
+            for (int64_t i = 0; i < m; ++i) f[(size_t)i] = (float)vs[k][i];
+            HIPCHECK(c, hipMemcpy((float*)vd[k] + first, f.data(), sizeof(float) * m, hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return START_OK;
+}
+
+// Movement slots, in load order, for the particles with load index >= first
+// that the next step moves (not settled, |v| >= 10 m/s, inside the expanded
+// bounding box), while fewer than trk_cap are tracked -- the reference's map
+// takes a particle while it holds fewer than maxParticles
+// (ParticleMovementTracker.cpp:11-12).  first = 0: a whole population (any row
+// order); first > 0: rows first..n-1 just appended, which hold load indices
+// first..n-1.
+int assign_track_slots(start_ctx* c, int64_t first)
+{
+    const int64_t n = c->n;
+    const int64_t m_old = (int64_t)c->trk_rows.size();
+    if (c->trk_cap <= m_old || first >= n) return START_OK;
+    uint8_t* d_el = nullptr;
+    HIPCHECK(c, dalloc(&d_el, n));
+    StepParams S = step_params(c);
+    if (c->mixed) {
+        DevParticles<float> P = c->view<float>();
+        if (first > 0)
+            P = DevParticles<float>{P.x + first, P.y + first, P.z + first, P.vx + first, P.vy + first, P.vz + first,
+                                    P.tet + first, P.flags + first, P.hit + first, P.pidx + first, n - first};
+        launch_track_eligible<float>(c->stream, c->mesh_view(), P, S, d_el);
+    } else {
+        DevParticles<double> P = c->view<double>();
+        if (first > 0)
+            P = DevParticles<double>{P.x + first, P.y + first, P.z + first, P.vx + first, P.vy + first,
+                                     P.vz + first, P.tet + first, P.flags + first, P.hit + first, P.pidx + first,
+                                     n - first};
+        launch_track_eligible<double>(c->stream, c->mesh_view(), P, S, d_el);
+    }
+    std::vector<uint8_t> el((size_t)(n - first));
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(el.data(), d_el + first, (size_t)(n - first), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_el);
+    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("movement tracking: ") + hipGetErrorString(e));
+    std::vector<int32_t> slot((size_t)(n - first), -1);
+    for (int64_t li = first; li < n && (int64_t)c->trk_rows.size() < c->trk_cap; ++li)
+        if (el[(size_t)(li - first)]) {
+            slot[(size_t)(li - first)] = (int32_t)c->trk_rows.size();
+            c->trk_rows.push_back(li);
+        }
+    const int64_t m_new = (int64_t)c->trk_rows.size();
+    if (m_new == m_old) return START_OK;
+    if (!c->d_trk_slot) {
+        e = dalloc(&c->d_trk_slot, std::max(c->cap, n));
+        if (e == hipSuccess) e = hipMemset(c->d_trk_slot, 0xff, sizeof(int32_t) * (size_t)std::max(c->cap, n));
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(c->d_trk_slot + first, slot.data(), sizeof(int32_t) * (size_t)(n - first), hipMemcpyHostToDevice);
+    int32_t* cnt2 = nullptr;
+    double* xyz2 = nullptr;
+    const size_t per = 3 * (size_t)c->trk_max;
+    if (e == hipSuccess) e = dalloc(&cnt2, m_new);
+    if (e == hipSuccess) e = dalloc(&xyz2, (size_t)m_new * per);
+    if (e == hipSuccess) e = hipMemset(cnt2, 0, sizeof(int32_t) * (size_t)m_new);
+    if (e == hipSuccess && m_old > 0) {
+        e = hipMemcpy(cnt2, c->d_trk_count, sizeof(int32_t) * (size_t)m_old, hipMemcpyDeviceToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(xyz2, c->d_trk_xyz, sizeof(double) * (size_t)m_old * per, hipMemcpyDeviceToDevice);
+    }
+    if (e != hipSuccess) {
+        if (cnt2) (void)hipFree(cnt2);
+        if (xyz2) (void)hipFree(xyz2);
+        c->trk_rows.resize((size_t)m_old);
+        return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("movement tracking: ") + hipGetErrorString(e));
+    }
+    if (c->d_trk_count) (void)hipFree(c->d_trk_count);
+    if (c->d_trk_xyz) (void)hipFree(c->d_trk_xyz);
+    c->d_trk_count = cnt2;
+    c->d_trk_xyz = xyz2;
+    drop_graphs(c);  // captured launches hold the record pointers
     return START_OK;
 }
 
@@ -841,10 +1051,6 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
     std::vector<float> g32;
     if (c->mixed) {
         c->tau32 = build_geo32(c->hm, g32);
-        if (!(c->tau32 < 0.01f)) {
-            delete c;
-            return fail(nullptr, START_ERR_MESH, "mesh too ill-conditioned for the FP32 barycentric fast path");
-        }
     }
 
     auto up = [&](void* dst, const void* src, size_t bytes) {
@@ -975,46 +1181,10 @@ int start_load_batch(start_ctx* c, int32_t n_runs, const start_run_desc* runs, c
     }
     int rc = begin_population(c, rk, n, ids);
     if (rc != START_OK || n == 0) return rc;
-    rc = init_population_tail(c, n);
+    int64_t nact = n;
+    HIPCHECK(c, hipMemcpy(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice));
+    rc = upload_rows(c, 0, n, x, y, z, vx, vy, vz, flags.data());
     if (rc != START_OK) return rc;
-    size_t b8 = sizeof(double) * (size_t)n;
-    HIPCHECK(c, hipMemcpy(c->A.flags, flags.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    DevMesh M = c->mesh_view();
-    if (!c->mixed) {
-        HIPCHECK(c, hipMemcpy(c->A.x, x, b8, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->A.y, y, b8, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->A.z, z, b8, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->A.vx, vx, b8, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->A.vy, vy, b8, hipMemcpyHostToDevice));
-        HIPCHECK(c, hipMemcpy(c->A.vz, vz, b8, hipMemcpyHostToDevice));
-        launch_locate(c->stream, M, (double*)c->A.x, (double*)c->A.y, (double*)c->A.z, n, c->A.tet, c->A.flags,
-                      nullptr, nullptr, nullptr, c->d_cnt);
-    } else {
-        // positions go to the frame of their owner; velocities to FP32
-        double *dx = nullptr, *dy = nullptr, *dz = nullptr;
-        hipError_t e = dalloc(&dx, n);
-        if (e == hipSuccess) e = dalloc(&dy, n);
-        if (e == hipSuccess) e = dalloc(&dz, n);
-        if (e == hipSuccess) e = hipMemcpy(dx, x, b8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dy, y, b8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dz, z, b8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            launch_locate(c->stream, M, dx, dy, dz, n, c->A.tet, c->A.flags, (float*)c->A.x, (float*)c->A.y,
-                          (float*)c->A.z, c->d_cnt);
-            e = hipStreamSynchronize(c->stream);
-        }
-        (void)hipFree(dx);
-        (void)hipFree(dy);
-        (void)hipFree(dz);
-        if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("particle upload: ") + hipGetErrorString(e));
-        std::vector<float> f((size_t)n);
-        const double* vs[3] = {vx, vy, vz};
-        void* vd[3] = {c->A.vx, c->A.vy, c->A.vz};
-        for (int k = 0; k < 3; ++k) {
-            for (int64_t i = 0; i < n; ++i) f[(size_t)i] = (float)vs[k][i];
-            HIPCHECK(c, hipMemcpy(vd[k], f.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-        }
-    }
     HIPCHECK(c, hipGetLastError());
     if (c->cfg.sort_interval > 0) {
         int rc2 = sort_now(c);
@@ -1159,6 +1329,43 @@ int start_generate(start_ctx* c, int32_t n_sources, const start_source_desc* src
         rc = sort_now(c);
         if (rc != START_OK) return rc;
     }
+    return check_kernel_errors(c);
+}
+
+int start_append_particles(start_ctx* c, int64_t m, const double* x, const double* y, const double* z,
+                           const double* vx, const double* vy, const double* vz, const int32_t* species,
+                           const uint64_t* ids)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (m < 0 || c->n + m > (int64_t)0x7fffffff) return fail(c, START_ERR_ARG, "particle count out of range");
+    if (m == 0) return START_OK;
+    if (c->n_runs() != 1) return fail(c, START_ERR_STATE, "particles can be appended to a single simulation only");
+    if (!x || !y || !z || !vx || !vy || !vz || !species) return fail(c, START_ERR_ARG, "null particle array");
+    std::vector<int32_t> flags((size_t)m);
+    for (int64_t i = 0; i < m; ++i) {
+        if (species[i] < 0 || species[i] >= (int32_t)c->species.size())
+            return fail(c, START_ERR_ARG, "species index out of range at particle " + std::to_string(i));
+        flags[(size_t)i] = species[i] << kSpeciesShift;  // run 0
+    }
+    const int64_t n0 = c->n;
+    int rc = grow_particles(c, n0 + m);
+    if (rc != START_OK) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    // launches read their first step from the device counter, which steps
+    // taken while the context held no particles did not advance
+    const unsigned long long step_now = (unsigned long long)c->step;
+    HIPCHECK(c, hipMemcpy(c->d_cnt + kCntStep + c->wslot, &step_now, sizeof(step_now), hipMemcpyHostToDevice));
+    rc = upload_rows(c, n0, m, x, y, z, vx, vy, vz, flags.data());
+    if (rc != START_OK) return rc;
+    c->n = n0 + m;
+    for (int64_t i = 0; i < m; ++i) c->ids.push_back(ids ? ids[i] : (uint64_t)(n0 + i));
+    // after a sort the settled particles close the live prefix; the kernels
+    // cover every row until the next sort compacts them again
+    int64_t nact = c->n;
+    HIPCHECK(c, hipMemcpy(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice));
+    drop_graphs(c);  // captured launches hold the old particle count
+    rc = assign_track_slots(c, n0);
+    if (rc != START_OK) return rc;
     return check_kernel_errors(c);
 }
 
@@ -1309,6 +1516,7 @@ int start_step(start_ctx* c, int32_t nsteps)
         c->step += (uint64_t)nsteps;
         return START_OK;
     }
+    if (!c->d_wl || !c->B.x) return fail(c, START_ERR_STATE, "particle work buffers missing (a failed append)");
     // Launches of steps_per_launch steps; before a launch, a sort if
     // sort_interval steps have passed since the last one.  A chunk is the run
     // of launches between two sorts.
@@ -1435,42 +1643,13 @@ int start_track_movements(start_ctx* c, int64_t max_particles, int32_t max_point
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);  // captured launches hold the record pointers
     free_tracking(c);
-    const int64_t n = c->n;
-    if (max_particles == 0 || n == 0) return START_OK;
+    if (max_particles == 0) return START_OK;
     // the first max_particles particles, in load order, that the next step
-    // records (the reference's map fills in its loop order while it has room)
-    uint8_t* d_el = nullptr;
-    HIPCHECK(c, dalloc(&d_el, n));
-    StepParams S = step_params(c);
-    if (c->mixed) launch_track_eligible<float>(c->stream, c->mesh_view(), c->view<float>(), S, d_el);
-    else launch_track_eligible<double>(c->stream, c->mesh_view(), c->view<double>(), S, d_el);
-    std::vector<uint8_t> el(n);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(el.data(), d_el, (size_t)n, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(d_el);
-    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("movement tracking: ") + hipGetErrorString(e));
-    std::vector<int32_t> slot(n, -1);
-    for (int64_t r = 0; r < n && (int64_t)c->trk_rows.size() < max_particles; ++r)
-        if (el[r]) {
-            slot[r] = (int32_t)c->trk_rows.size();
-            c->trk_rows.push_back(r);
-        }
-    const int64_t m = (int64_t)c->trk_rows.size();
-    if (m == 0) return START_OK;
-    e = dalloc(&c->d_trk_slot, n);
-    if (e == hipSuccess) e = dalloc(&c->d_trk_count, m);
-    if (e == hipSuccess) e = dalloc(&c->d_trk_xyz, 3 * (size_t)m * (size_t)max_points);
-    if (e == hipSuccess)
-        e = hipMemcpy(c->d_trk_slot, slot.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(c->d_trk_count, 0, sizeof(int32_t) * (size_t)m);
-    if (e != hipSuccess) {
-        free_tracking(c);
-        return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
-                    std::string("movement tracking: ") + hipGetErrorString(e));
-    }
+    // records (the reference's map fills in its loop order while it has room);
+    // particles appended later join while there is room
+    c->trk_cap = max_particles;
     c->trk_max = max_points;
-    return START_OK;
+    return assign_track_slots(c, 0);
 }
 
 int start_read_movements(start_ctx* c, int64_t* n, uint64_t* ids, int32_t* counts, double* xyz)
@@ -1512,9 +1691,9 @@ int start_record_settle(start_ctx* c, int32_t on)
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);  // captured launches hold the record pointer
     c->record = on != 0;
-    if (c->record && !c->d_settle && c->n > 0) {
-        HIPCHECK(c, dalloc(&c->d_settle, c->n));
-        HIPCHECK(c, hipMemset(c->d_settle, 0xff, sizeof(SettleRec) * c->n));
+    if (c->record && !c->d_settle && c->n > 0) {  // one record per load index, appends included
+        HIPCHECK(c, dalloc(&c->d_settle, c->cap));
+        HIPCHECK(c, hipMemset(c->d_settle, 0xff, sizeof(SettleRec) * c->cap));
     }
     return START_OK;
 }
@@ -1837,6 +2016,27 @@ int start_locate_noalloc(start_ctx* c, int64_t n, const double* x, const double*
     HIPCHECK(c, hipMemcpyAsync(out_tet, dt, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return check_kernel_errors(c);
+}
+
+int start_mesh_maps(const start_mesh_desc* mesh, int32_t* tets_out, double* geo, double* rho, double* tau,
+                    float* geo32, double* tau32)
+{
+    if (!mesh || !geo || !tau) return fail(nullptr, START_ERR_ARG, "null argument");
+    HostMesh hm;
+    std::string e = prepare_mesh(hm, mesh->n_nodes, mesh->nodes, mesh->n_tets, mesh->tets, mesh->n_tris, mesh->tris,
+                                 1.01);
+    if (!e.empty()) return fail(nullptr, START_ERR_MESH, e);
+    std::memcpy(geo, hm.geo.data(), sizeof(double) * hm.geo.size());
+    if (tets_out) std::memcpy(tets_out, hm.tets.data(), sizeof(int32_t) * hm.tets.size());
+    if (rho) std::memcpy(rho, hm.rho.data(), sizeof(double) * hm.rho.size());
+    *tau = hm.tau;
+    if (geo32 || tau32) {
+        std::vector<float> g32;
+        const float t32 = build_geo32(hm, g32);
+        if (geo32) std::memcpy(geo32, g32.data(), sizeof(float) * g32.size());
+        if (tau32) *tau32 = (double)t32;
+    }
+    return START_OK;
 }
 
 int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign)

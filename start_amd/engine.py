@@ -93,6 +93,22 @@ class ParticleInCellStep:
         self.n = len(pos)
         self.n_runs = 1
 
+    def append_particles(self, pos: np.ndarray, vel: np.ndarray, species: np.ndarray, ids=None):
+        """Appends particles after the loaded ones, keeping everything loaded
+        (the continuous-sputtering spawn, ModelingMainDriver.cpp:141-142)."""
+        pos = np.asarray(pos, dtype=np.float64)
+        vel = np.asarray(vel, dtype=np.float64)
+        cols = [np.ascontiguousarray(a) for a in (pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0],
+                                                  vel[:, 1], vel[:, 2])]
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        idp = None
+        if ids is not None:
+            ids = np.ascontiguousarray(ids, dtype=np.uint64)
+            idp = _up(ids)
+        self._check(_lib.lib().start_append_particles(self.ctx, len(pos), *[_dp(c) for c in cols],
+                                                      _ip(sp), idp))
+        self.n += len(pos)
+
     def generate(self, sources: list[dict], seed: int = 0):
         """Particles drawn on the device from point / surface sources (replaces
         ParticleGeneratorHost / ParticleGeneratorDevice and the displacer).
@@ -338,6 +354,26 @@ class ParticleInCellStep:
         n = C.c_uint64(0)
         self._check(_lib.lib().start_timing_read(self.ctx, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def mesh_maps(mesh: TetMesh) -> dict:
+    """The barycentric maps and certification thresholds the device uses
+    (computed on the host; no GPU needed): positively oriented tets, geo
+    (n_tets x 12: v0, A row-major), rho (per tet: bound on the row sums of
+    |A E - I|), tau, geo32 (n_tets x 12: A, v0) and tau32."""
+    nodes = np.ascontiguousarray(mesh.nodes, dtype=np.float64)
+    tets = np.ascontiguousarray(mesh.tets, dtype=np.int32)
+    tris = np.ascontiguousarray(mesh.tris, dtype=np.int32)
+    md = _lib.MeshDesc(len(nodes), _dp(nodes), len(tets), _ip(tets), len(tris), _ip(tris))
+    T = len(tets)
+    out_tets = np.empty((T, 4), dtype=np.int32)
+    geo = np.empty((T, 12), dtype=np.float64)
+    rho = np.empty(T, dtype=np.float64)
+    geo32 = np.empty((T, 12), dtype=np.float32)
+    tau, tau32 = C.c_double(0), C.c_double(0)
+    _lib.check(_lib.lib().start_mesh_maps(C.byref(md), _ip(out_tets), _dp(geo), _dp(rho), C.byref(tau),
+                                          geo32.ctypes.data_as(C.POINTER(C.c_float)), C.byref(tau32)), None)
+    return {"tets": out_tets, "geo": geo, "rho": rho, "tau": tau.value, "geo32": geo32, "tau32": tau32.value}
 
 
 def orient3d_batch(pts: np.ndarray) -> np.ndarray:

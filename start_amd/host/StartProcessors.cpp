@@ -176,13 +176,20 @@ void ParticleKernelDevice::configure(std::string_view scatteringModel, std::stri
 
 void ParticleKernelDevice::ensureParticles(const ParticleVector& particles)
 {
-    if (m_n == particles.size() && m_n > 0) return;
-    size_t n = particles.size();
-    std::vector<double> x(n), y(n), z(n), vx(n), vy(n), vz(n);
-    std::vector<int32_t> sp(n);
-    std::vector<uint64_t> ids(n);
-    for (size_t i = 0; i < n; ++i) {
-        const Particle& p = particles[i];
+    const size_t n = particles.size();
+    if (m_loaded && n == m_n) return;
+    // The device keeps every particle it was given, settled ones included, and
+    // the reference only ever inserts new particles at the end of the vector
+    // (ModelingMainDriver.cpp:64-67): any other change of size is an error.
+    if (m_loaded && n < m_n)
+        throw std::runtime_error("the particle vector shrank from " + std::to_string(m_n) + " to " +
+                                 std::to_string(n) + " particles; only appending at the end is supported");
+    const size_t first = m_loaded ? m_n : 0, m = n - first;
+    std::vector<double> x(m), y(m), z(m), vx(m), vy(m), vz(m);
+    std::vector<int32_t> sp(m);
+    std::vector<uint64_t> ids(m);
+    for (size_t i = 0; i < m; ++i) {
+        const Particle& p = particles[first + i];
         if (p.getType() == Unknown) throw std::runtime_error("particle of unknown type");
         auto c = p.getCentre();
         auto v = p.getVelocity();
@@ -195,13 +202,21 @@ void ParticleKernelDevice::ensureParticles(const ParticleVector& particles)
         sp[i] = (int32_t)p.getType();
         ids[i] = (uint64_t)p.getId();
     }
-    check(start_load_particles(m_ctx, (int64_t)n, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
-                               sp.data(), ids.data()),
-          "particle upload");
+    if (!m_loaded) {
+        check(start_load_particles(m_ctx, (int64_t)m, x.data(), y.data(), z.data(), vx.data(), vy.data(), vz.data(),
+                                   sp.data(), ids.data()),
+              "particle upload");
+        m_settled_seen = 0;
+        m_tracking = false;  // a new population ends the device's records
+    } else {
+        // spawned particles join the population; everything loaded is kept
+        check(start_append_particles(m_ctx, (int64_t)m, x.data(), y.data(), z.data(), vx.data(), vy.data(),
+                                     vz.data(), sp.data(), ids.data()),
+              "particle append");
+    }
     m_n = n;
-    m_settled_seen = 0;
-    m_charge_fresh = false;
-    m_tracking = false;  // a new population ends the device's records
+    m_loaded = true;
+    m_charge_fresh = false;  // the node charge does not hold the new particles yet
 }
 
 void ParticleKernelDevice::startMovements()

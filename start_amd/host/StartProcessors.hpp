@@ -141,6 +141,9 @@ private:
     void ensureContext();
     void configure(std::string_view scatteringModel, std::string_view gasName, double gasConcentration,
                    double timeStep);
+    // Loads the ParticleVector at the first call; afterwards appends the
+    // particles inserted at its end (continuous sputtering spawns every step)
+    // and throws on any other change of size.
     void ensureParticles(const ParticleVector& particles);
     void check(int rc, const char* what);
     void step(ParticleVector& particles, ParticlesIDSet& settled, StopSubject& stop);
@@ -159,6 +162,7 @@ private:
     double m_conc = 0.0;
     std::vector<double> m_phi_rows, m_E;  // field given before the context existed
     size_t m_n = 0;
+    bool m_loaded = false;  // the device holds the population (appends extend it)
     uint64_t m_settled_seen = 0;
     int64_t m_steps = 0;
     bool m_charge_fresh = false;  // device node charge belongs to the current positions

@@ -10,7 +10,13 @@
 //   config.txt    one "key value" per line: dt, steps, model, gas, pressure,
 //                 temperature, seed, precision (fp64|mixed), density
 //                 (barycentric|tet_average), em_push (0|1), diameter_<Type>,
-//                 charge_<Type> (in units of e)
+//                 charge_<Type> (in units of e); optional spawn_every m and
+//                 spawn_count k: at the start of every m-th step after the
+//                 first, k particles of spawn_pos.f64 (3K) spawn_vel.f64 (3K)
+//                 spawn_type.i32 (K) spawn_id.u64 (K), in order, are inserted
+//                 at the end of the ParticleVector (continuous sputtering,
+//                 ModelingMainDriver.cpp:37-71, :141-142); optional shrink_at s:
+//                 the last particle is removed before step s (an error)
 // Outputs in <dir>:
 //   out_pos.f64 out_vel.f64  particle state after the loop (ParticleVector order)
 //   out_settled.u64           settled ids, ascending
@@ -92,6 +98,20 @@ int main(int argc, char** argv)
             particles.emplace_back((size_t)id[i], (ParticleType)type[i], pos[3 * i], pos[3 * i + 1], pos[3 * i + 2],
                                    vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
 
+        const int spawn_every = cfg.count("spawn_every") ? std::stoi(cfg.at("spawn_every")) : 0;
+        const size_t spawn_count = cfg.count("spawn_count") ? (size_t)std::stoul(cfg.at("spawn_count")) : 0;
+        const int shrink_at = cfg.count("shrink_at") ? std::stoi(cfg.at("shrink_at")) : -1;
+        std::vector<double> spos, svel;
+        std::vector<int32_t> stype;
+        std::vector<uint64_t> sid;
+        if (spawn_every > 0) {
+            spos = read_array<double>(dir + "/spawn_pos.f64");
+            svel = read_array<double>(dir + "/spawn_vel.f64");
+            stype = read_array<int32_t>(dir + "/spawn_type.i32");
+            sid = read_array<uint64_t>(dir + "/spawn_id.u64");
+        }
+        size_t spawned = 0;
+
         const double dt = num("dt");
         const int steps = std::stoi(cfg.at("steps"));
         const double conc = start_gas_concentration(num("pressure"), num("temperature"));
@@ -105,6 +125,12 @@ int main(int argc, char** argv)
         int k = 0;
         double t = 0.0;
         for (; k < steps && !stop.stopRequested(); ++k, t += dt) {
+            if (spawn_every > 0 && k > 0 && k % spawn_every == 0)
+                for (size_t j = 0; j < spawn_count && spawned < sid.size(); ++j, ++spawned)
+                    particles.emplace_back((size_t)sid[spawned], (ParticleType)stype[spawned], spos[3 * spawned],
+                                           spos[3 * spawned + 1], spos[3 * spawned + 2], svel[3 * spawned],
+                                           svel[3 * spawned + 1], svel[3 * spawned + 2]);
+            if (k == shrink_at) particles.pop_back();
             NodeChargeDensityProcessor::gather(t, dir + "/config.txt", device, particles, settled, tracker, density);
             ParticleDynamicsProcessor::process(dir + "/config.txt", particles, t, dt, 16, device, tracker, settled,
                                                sh, mx, movements, stop, cfg.at("model"), cfg.at("gas"), conc);
@@ -145,7 +171,8 @@ int main(int argc, char** argv)
         }
         write_array(dir + "/out_density.f64", rho);
         write_array(dir + "/out_density_written.i32", wr);
-        std::ofstream(dir + "/out_summary.txt") << "steps " << k << "\nstop " << (stop.stopRequested() ? 1 : 0) << "\n";
+        std::ofstream(dir + "/out_summary.txt") << "steps " << k << "\nstop " << (stop.stopRequested() ? 1 : 0)
+                                                << "\nparticles " << particles.size() << "\n";
         std::cout << "shim_check: " << k << " steps, " << settled.size() << " settled" << std::endl;
     } catch (std::exception const& ex) {
         std::cerr << "shim_check: " << ex.what() << std::endl;

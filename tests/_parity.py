@@ -52,6 +52,18 @@ class Pair:
         self.step = 0
         self.processed_o = 0
 
+    def append(self, pos, vel, sp):
+        """Appends particles to both sides: the oracle's state grows with load
+        indices n.., as the device's start_append_particles numbers them."""
+        n0 = len(self.st.x)
+        tet = self.om.locate_many(pos[:, 0], pos[:, 1], pos[:, 2])
+        new = O.State.new(pos, vel, sp, np.arange(n0, n0 + len(pos), dtype=np.uint64), tet)
+        for f in self.st.__dataclass_fields__:
+            setattr(self.st, f, np.ascontiguousarray(np.concatenate([getattr(self.st, f), getattr(new, f)])))
+        self.tet0 = np.concatenate([self.tet0, tet])
+        self.eng.append_particles(pos, vel, sp)
+        self.n += len(pos)
+
     def advance(self, k):
         """k steps; the node charge of a launch of K fused steps is the sum of
         the per-step charges of those steps."""

@@ -22,24 +22,34 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TYPES = ["O2", "Ar", "Ne", "He", "Ti", "Al", "Sn", "W", "Au", "Cu", "Ni", "Ag"]
 
 
-def _run_shim(tmp, w, steps, density, charge_units, diameter):
-    n = len(w.pos)
+def _run_shim(tmp, w, steps, density, charge_units, diameter, n=None, extra=None, spawn=None):
+    """shim_check on the first n particles of w; spawn = (pos, vel) of the
+    particles inserted later (extra: spawn_every / spawn_count / shrink_at)."""
+    n = len(w.pos) if n is None else n
     tmp.mkdir(exist_ok=True)
     w.mesh.nodes.astype(np.float64).tofile(tmp / "nodes.f64")
     w.mesh.tets.astype(np.int32).tofile(tmp / "tets.i32")
     w.mesh.tris.astype(np.int32).tofile(tmp / "tris.i32")
     w.phi.astype(np.float64).tofile(tmp / "phi.f64")
-    w.pos.astype(np.float64).tofile(tmp / "pos.f64")
-    w.vel.astype(np.float64).tofile(tmp / "vel.f64")
+    w.pos[:n].astype(np.float64).tofile(tmp / "pos.f64")
+    w.vel[:n].astype(np.float64).tofile(tmp / "vel.f64")
     np.full(n, TYPES.index("Ti"), dtype=np.int32).tofile(tmp / "type.i32")
     np.arange(1000, 1000 + n, dtype=np.uint64).tofile(tmp / "id.u64")
+    if spawn is not None:
+        sp_pos, sp_vel = spawn
+        sp_pos.astype(np.float64).tofile(tmp / "spawn_pos.f64")
+        sp_vel.astype(np.float64).tofile(tmp / "spawn_vel.f64")
+        np.full(len(sp_pos), TYPES.index("Ti"), dtype=np.int32).tofile(tmp / "spawn_type.i32")
+        np.arange(1000 + n, 1000 + n + len(sp_pos), dtype=np.uint64).tofile(tmp / "spawn_id.u64")
     cfg = {"dt": repr(w.dt), "steps": steps, "model": "hs", "gas": "Ar", "pressure": 0.5,
            "temperature": 300.0, "seed": w.seed, "precision": "fp64", "density": density,
-           "diameter_Ti": repr(diameter), "charge_Ti": charge_units}
+           "diameter_Ti": repr(diameter), "charge_Ti": charge_units, **(extra or {})}
     (tmp / "config.txt").write_text("".join(f"{k} {v}\n" for k, v in cfg.items()))
-    log(f"shim_check: {n} particles, {steps} steps, density {density}")
+    log(f"shim_check: {n} particles, {steps} steps, density {density}, {extra or ''}")
     r = subprocess.run([os.path.join(ROOT, "start_amd", "shim_check"), str(tmp)], capture_output=True,
                        text=True, timeout=300)
+    if extra and "shrink_at" in extra:
+        return {"rc": r.returncode, "stdout": r.stdout, "stderr": r.stderr}
     assert r.returncode == 0 and "ERROR" not in r.stderr, r.stdout + r.stderr
     return {
         "pos": np.fromfile(tmp / "out_pos.f64").reshape(-1, 3),
@@ -55,38 +65,51 @@ def _run_shim(tmp, w, steps, density, charge_units, diameter):
     }
 
 
-@pytest.mark.parametrize("density", ["barycentric", "tet_average"])
-def test_cpp_step_loop_matches_oracle(tmp_path, density):
-    charge_units, diameter, steps = 2.0, 2.0e-10, 60
-    w = workloads.small_box(n_particles=2500, charge_units=charge_units)
-    out = _run_shim(tmp_path / density, w, steps, density, charge_units, diameter)
-    # the oracle with one species per particle type, as the C++ side sets them up
+def _oracle_loop(w, steps, charge_units, diameter, n, spawn_every=0, spawn=None, spawn_count=0):
+    """The oracle stepped as shim_check drives the C++ loop: particles spawned
+    at the start of every spawn_every-th step after the first, then one step.
+    Returns the state, the simulation, the last step's node charge and the
+    particleMovementMap the reference fills (every moving particle: capacity
+    200,000)."""
     om = O.Mesh(w.mesh.nodes, w.mesh.tets, w.mesh.tris)
     E = om.tet_field(w.phi)
     gas_mass = S.MASS["Ar"]
     ks = []
-    for t in TYPES:
+    for t in TYPES:  # one species per particle type, as the C++ side sets them up
         q = charge_units * S.E_CHARGE if t == "Ti" else S.CHARGE_UNITS.get(t, 0) * S.E_CHARGE
         d = diameter if t == "Ti" else S.RADIUS[t] + S.RADIUS["Ar"]
         ks.append(O.species_constants(S.MASS[t], q, d, gas_mass, S.VTI["Ar"], S.VSS_ALPHA["Ar"], O.MODEL["HS"]))
-    n = len(w.pos)
-    st = O.State.new(w.pos, w.vel, np.full(n, TYPES.index("Ti")), np.arange(n, dtype=np.uint64),
-                     om.locate_many(w.pos[:, 0], w.pos[:, 1], w.pos[:, 2]))
+    st = O.State.new(w.pos[:n], w.vel[:n], np.full(n, TYPES.index("Ti")), np.arange(n, dtype=np.uint64),
+                     om.locate_many(w.pos[:n, 0], w.pos[:n, 1], w.pos[:n, 2]))
     sim = O.Sim(om, ks, O.MODEL["HS"], S.concentration(0.5, 300.0), w.dt, w.seed, E=E)
     q = np.zeros(w.mesh.n_nodes)
-    moved = np.zeros(n, dtype=np.uint8)
-    expect = None  # particleMovementMap as the reference fills it (all 2,500 are tracked)
+    expect = {}  # particleMovementMap: a particle joins when it first moves
+    spawned = 0
     for k in range(steps):
+        if spawn_every and k > 0 and k % spawn_every == 0 and spawned < len(spawn[0]):
+            m = min(spawn_count, len(spawn[0]) - spawned)
+            pos, vel = spawn[0][spawned:spawned + m], spawn[1][spawned:spawned + m]
+            n0 = len(st.x)
+            new = O.State.new(pos, vel, np.full(m, TYPES.index("Ti")), np.arange(n0, n0 + m, dtype=np.uint64),
+                              om.locate_many(pos[:, 0], pos[:, 1], pos[:, 2]))
+            for f in st.__dataclass_fields__:
+                setattr(st, f, np.ascontiguousarray(np.concatenate([getattr(st, f), getattr(new, f)])))
+            spawned += m
         q[:] = 0.0
         x0 = np.stack([st.x, st.y, st.z], 1)
         settled0 = st.status == O.SETTLED
+        moved = np.zeros(len(st.x), dtype=np.uint8)
         sim.step(st, k, q, nthreads=8, moved=moved)
-        if expect is None:
-            expect = {int(i): [] for i in np.nonzero(moved)[0]}
         for i in np.nonzero(moved)[0]:
-            expect[int(i)].append(("start", x0[i]))
+            pts = expect.setdefault(int(i), [])
+            pts.append(("start", x0[i]))
             if not settled0[i] and st.status[i] == O.SETTLED:
-                expect[int(i)].append(("hit", x0[i], np.array([st.x[i], st.y[i], st.z[i]]), int(st.hit[i])))
+                pts.append(("hit", x0[i], np.array([st.x[i], st.y[i], st.z[i]]), int(st.hit[i])))
+    return om, st, sim, q, expect
+
+
+def _check(out, w, om, st, sim, q, expect, density, steps):
+    """The outputs of shim_check against the oracle's loop."""
     assert out["summary"].startswith(f"steps {steps}")
     # the movement map of process()
     assert np.array_equal(out["mv_ids"], np.array(sorted(expect), dtype=np.uint64) + 1000)
@@ -101,6 +124,7 @@ def test_cpp_step_loop_matches_oracle(tmp_path, density):
                 check_hit(w.mesh, p, e[1], e[2], e[3], 1e-9)
     xo = np.stack([st.x, st.y, st.z], 1)
     vo = np.stack([st.vx, st.vy, st.vz], 1)
+    assert out["pos"].shape == xo.shape
     ex = np.abs(out["pos"] - xo).max(1) / np.linalg.norm(xo, axis=1)
     ev = np.abs(out["vel"] - vo).max(1) / np.maximum(np.linalg.norm(vo, axis=1), 1e-300)
     assert ex.max() <= REL_XV and ev.max() <= REL_XV, (ex.max(), ev.max())
@@ -116,5 +140,45 @@ def test_cpp_step_loop_matches_oracle(tmp_path, density):
     assert np.array_equal(out["written"], wr)
     err = np.abs(out["rho"] - ref) / np.maximum(np.abs(ref), floor)
     assert err.max() <= REL_Q, err.max()
-    log(f"shim {density}: ok, settled {len(settled_o)}, max rel x {ex.max():.2e} v {ev.max():.2e} "
-        f"rho {err.max():.2e}")
+    return ex.max(), ev.max(), err.max(), len(settled_o)
+
+
+@pytest.mark.parametrize("density", ["barycentric", "tet_average"])
+def test_cpp_step_loop_matches_oracle(tmp_path, density):
+    charge_units, diameter, steps = 2.0, 2.0e-10, 60
+    w = workloads.small_box(n_particles=2500, charge_units=charge_units)
+    out = _run_shim(tmp_path / density, w, steps, density, charge_units, diameter)
+    om, st, sim, q, expect = _oracle_loop(w, steps, charge_units, diameter, len(w.pos))
+    ex, ev, eq, ns = _check(out, w, om, st, sim, q, expect, density, steps)
+    log(f"shim {density}: ok, settled {ns}, max rel x {ex:.2e} v {ev:.2e} rho {eq:.2e}")
+
+
+@pytest.mark.parametrize("density", ["barycentric", "tet_average"])
+def test_cpp_step_loop_with_spawning_matches_oracle(tmp_path, density):
+    """Continuous sputtering: k particles inserted at the end of the vector
+    every m steps (ModelingMainDriver.cpp:141-142).  The device keeps what it
+    has and appends the new ones: settled ids, triangle counts, owners and the
+    movement map match the oracle running the same spawns."""
+    charge_units, diameter, steps, every, count = 2.0, 2.0e-10, 80, 7, 150
+    w = workloads.small_box(n_particles=4000, charge_units=charge_units)
+    n = 1200
+    spawn = (w.pos[n:], w.vel[n:])
+    out = _run_shim(tmp_path / density, w, steps, density, charge_units, diameter, n=n,
+                    extra={"spawn_every": every, "spawn_count": count}, spawn=spawn)
+    om, st, sim, q, expect = _oracle_loop(w, steps, charge_units, diameter, n, every, spawn, count)
+    n_total = n + count * ((steps - 1) // every)
+    assert len(st.x) == n_total and f"particles {n_total}" in out["summary"]
+    ex, ev, eq, ns = _check(out, w, om, st, sim, q, expect, density, steps)
+    # settled particles among the early ones and the spawned ones
+    assert (st.status[:n] == O.SETTLED).any() and (st.status[n:] == O.SETTLED).any()
+    log(f"shim spawning {density}: ok, {n_total} particles, settled {ns}, max rel x {ex:.2e} v {ev:.2e} "
+        f"rho {eq:.2e}")
+
+
+def test_cpp_step_loop_rejects_a_shrinking_vector(tmp_path):
+    """Removing a particle from the vector is not something the reference
+    loop does; the device keeps every particle, so it is reported, not
+    silently reloaded."""
+    w = workloads.small_box(n_particles=500, charge_units=2.0)
+    r = _run_shim(tmp_path / "shrink", w, 10, "barycentric", 2.0, 2.0e-10, extra={"shrink_at": 4})
+    assert r["rc"] == 1 and "shrank from 500 to 499" in r["stderr"], r

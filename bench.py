@@ -11,8 +11,21 @@ A step is one pass of the hot path over all particles: push, move, collision,
 relocation, surface hit/settle, charge scatter.  The timed region holds exactly
 --steps steps with the particles resident in HBM, bracketed by device
 synchronisation.  value = particle-steps processed (not settled, not frozen)
-per second, in the headline mode (FP64, bit-exact with the CPU oracle); the
-other modes are reported under "modes".
+per second in the headline mode: FP64, one step per launch with the node
+charge of every step, the shape of the reference's loop (gather, solve,
+process every step), bit-exact with the CPU oracle.  The other modes (mixed
+precision, many steps fused per launch) are reported under "modes".
+
+roofline: the streaming per-step kernel is bound by HBM; achieved = the
+algorithmic bytes of its launches (296 B per FP64 particle-step, 188 B mixed)
+over the median HIP-event duration of >= 5 launches on the library's stream.
+Fused launches keep the particle state in registers across steps and are
+bound by VALU issue, not HBM: their roofline uses the VALU instructions per
+particle-step counted by rocprofv3 (profiles/traffic.json) against the issue
+rate of the chip.  traffic = the copy-calibrated HBM bytes per particle-step
+of the same kernel and launch shape (profiles/traffic.json) times the
+particle-steps of the timed launch.  A fraction above 1 is reported as
+invalid and fails the run.
 
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
@@ -30,18 +43,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+# VALU issue rate: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4
+# cycles (16 lanes) at the 2.4 GHz peak engine clock, in wave-instructions/s
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
 # algorithmic bytes per particle-step of the streaming step:
 #   2*(6s + 8) particle state + (3 s_E + 12 s_G + 16 + 16) tet record + 4*8 charge
 BYTES_PER_PS = {"fp64": 296, "mixed": 188}
 
 MODES = {
     # name: (precision, steps fused per launch, sort interval)
-    "fp64_fused": ("fp64", 200, 200),
     "fp64_per_step": ("fp64", 1, 50),
-    "mixed_fused": ("mixed", 200, 200),
     "mixed_per_step": ("mixed", 1, 50),
+    "fp64_fused": ("fp64", 200, 200),
+    "mixed_fused": ("mixed", 200, 200),
 }
-HEADLINE = "fp64_fused"
+HEADLINE = "fp64_per_step"
+MIN_TIMED_LAUNCHES = 5
 
 
 def log(msg):
@@ -57,6 +74,20 @@ def cpu_threads() -> int:
     if env and env.isdigit():
         n = min(n, int(env))
     return max(1, n)
+
+
+def timed_launches(eng, K: int, launches: int):
+    """Runs `launches` launches of K steps with HIP events on the library's
+    stream; returns (median, mean ms per launch of step_main, particle-steps
+    per launch)."""
+    s1 = eng.stats()
+    eng.timing(True)
+    eng.process(K * launches)
+    eng.synchronize()
+    ms = eng.timing_read_each()
+    eng.timing(False)
+    s2 = eng.stats()
+    return float(np.median(ms)), float(np.mean(ms)), (s2["processed"] - s1["processed"]) / max(len(ms), 1), len(ms)
 
 
 def gpu_run(w, mode: str, steps: int, warmup: int):
@@ -78,12 +109,10 @@ def gpu_run(w, mode: str, steps: int, warmup: int):
     eng.synchronize()
     t1 = time.perf_counter()
     s1 = eng.stats()
-    # kernel duration of step_main with HIP events on the library's stream
-    tsteps = min(steps, max(K, 200 if K == 1 else 2 * K))
-    eng.timing(True)
-    eng.process(tsteps)
-    eng.synchronize()
-    kern_ms, launches = eng.timing_read()
+    # duration of the streaming kernel: median over >= 5 launches (one step
+    # each, or K fused steps each), events on the library's stream
+    n_launch = max(MIN_TIMED_LAUNCHES, 200 if K == 1 else MIN_TIMED_LAUNCHES)
+    med_ms, mean_ms, ps_launch, n_timed = timed_launches(eng, K, n_launch)
     s2 = eng.stats()
     eng.close()
     processed = s1["processed"] - s0["processed"]
@@ -92,9 +121,56 @@ def gpu_run(w, mode: str, steps: int, warmup: int):
         "precision": prec, "steps_per_launch": K, "sort_interval": sort,
         "processed": processed, "wall_s": wall, "ps_per_s": processed / wall,
         "ms_per_step": 1e3 * wall / max(steps, 1),
-        "kernel_ms_per_launch": kern_ms / max(launches, 1), "launches_timed": launches,
-        "processed_per_launch": (s2["processed"] - s1["processed"]) / max(launches, 1),
+        "launch_ms_median": med_ms, "launch_ms_mean": mean_ms, "launches_timed": n_timed,
+        "processed_per_launch": ps_launch, "total_processed": s2["processed"],
         "deferred": s1["deferred"] - s0["deferred"], "settled": s1["settled"],
+    }
+
+
+def accuracy_run(w, steps: int):
+    """The mixed path against FP64 on the same inputs and variates: after
+    `steps` single-step launches of each, per-particle relative errors of x and
+    v (|d|_inf / |ref|), over the particles whose owner and status agree, and
+    the fraction whose owner or status differs."""
+    from start_amd.engine import ParticleInCellStep
+    out = {}
+    for prec in ("fp64", "mixed"):
+        eng = ParticleInCellStep(w.mesh, w.species, w.gas, w.dt, w.seed, precision=prec,
+                                 charge_every_step=True, steps_per_launch=1, sort_interval=50)
+        eng.load_particles(w.pos, w.vel, w.sp_idx)
+        eng.set_node_potential(w.phi)
+        eng.process(steps)
+        out[prec] = eng.particles()
+        out[prec]["charge"] = eng.node_charge()
+        eng.close()
+    a, b = out["fp64"], out["mixed"]
+    agree = (a["tet"] == b["tet"]) & (a["status"] == b["status"])
+    xa = np.stack([a["x"], a["y"], a["z"]], 1)
+    xb = np.stack([b["x"], b["y"], b["z"]], 1)
+    va = np.stack([a["vx"], a["vy"], a["vz"]], 1)
+    vb = np.stack([b["vx"], b["vy"], b["vz"]], 1)
+    ex = (np.abs(xa - xb).max(1) / np.linalg.norm(xa, axis=1))[agree]
+    vn = np.linalg.norm(va, axis=1)
+    ev = (np.abs(va - vb).max(1) / np.maximum(vn, 1e-300))[agree]
+    # against the thermal speed scale: the post-collision centre-of-mass sum
+    # (HSModel.cpp:25-34) nearly cancels for a few particles, whose |v| is then
+    # far below the scale of the arithmetic that produced it
+    v_scale = float(np.sqrt(np.mean(vn ** 2)))
+    ev_scale = (np.abs(va - vb).max(1) / v_scale)[agree]
+    qa, qb = a["charge"], b["charge"]
+    big = np.abs(qa) > 1e-3 * np.abs(qa).max()
+    slow = vn[agree][ev > 1e-5]
+    return {
+        "steps": steps, "particles": len(xa), "compared": int(agree.sum()),
+        "owner_or_status_differ": int((~agree).sum()),
+        "owner_or_status_differ_frac": float((~agree).mean()),
+        "settled_fp64": int((a["status"] == 1).sum()), "settled_mixed": int((b["status"] == 1).sum()),
+        "max_rel_x": float(ex.max()), "p999_rel_x": float(np.quantile(ex, 0.999)),
+        "max_rel_v": float(ev.max()), "p999_rel_v": float(np.quantile(ev, 0.999)),
+        "particles_rel_v_above_1e-5": int((ev > 1e-5).sum()),
+        "max_speed_of_those_m_s": float(slow.max()) if len(slow) else None,
+        "max_rel_v_of_thermal_speed": float(ev_scale.max()), "thermal_speed_m_s": v_scale,
+        "node_charge_max_rel": float((np.abs(qa - qb)[big] / np.abs(qa[big])).max()),
     }
 
 
@@ -137,17 +213,14 @@ def sweep_run(steps: int, warmup: int, trial_steps: int = 200):
         s1 = eng.stats()
         hists[prec] = eng.tri_counts_runs().astype(np.float64)
         K = eng.get_launch()["steps_per_launch"]
-        eng.timing(True)  # kernel time of step_main, outside the timed region
-        eng.process(max(K, 2 * K))
-        eng.synchronize()
-        kern_ms, launches = eng.timing_read()
-        s2 = eng.stats()
+        # kernel time of step_main, outside the timed region: >= 5 launches
+        med_ms, mean_ms, ps_launch, n_timed = timed_launches(eng, K, MIN_TIMED_LAUNCHES)
         eng.close()
         processed = s1["processed"] - s0["processed"]
         out[prec] = {"processed": processed, "wall_s": t1 - t0, "ps_per_s": processed / (t1 - t0),
-                     "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms_per_launch": kern_ms / max(launches, 1),
-                     "processed_per_launch": (s2["processed"] - s1["processed"]) / max(launches, 1),
-                     "steps_per_launch": K}
+                     "ms_per_step": 1e3 * (t1 - t0) / steps, "launch_ms_median": med_ms,
+                     "launch_ms_mean": mean_ms, "launches_timed": n_timed,
+                     "processed_per_launch": ps_launch, "steps_per_launch": K, "precision": prec}
         log(f"[bench] sweep {prec}: {out[prec]}")
     h64 = hists["fp64"]
     out["deposition_l1_vs_fp64"] = float(np.abs(hists["mixed"] - h64).sum() / max(h64.sum(), 1.0))
@@ -190,18 +263,21 @@ def chamber_run(steps: int):
         t1 = time.perf_counter()
         s1 = eng.stats()
         hists[prec] = eng.tri_counts()[sub].astype(np.float64)
-        eng.timing(True)  # kernel time of step_main, outside the timed region
+        # kernel time of step_main, outside the timed region: 10 launches,
+        # each after a field refresh as in the timed loop
+        eng.timing(True)
         for k in range(steps, steps + 10 * w.refresh, w.refresh):
             eng.set_node_potential(w.potential(k))
             eng.process(w.refresh)
         eng.synchronize()
-        kern_ms, launches = eng.timing_read()
+        ms = eng.timing_read_each()
         s2 = eng.stats()
         eng.close()
         processed = s1["processed"] - s0["processed"]
         out[prec] = {"processed": processed, "wall_s": t1 - t0, "ps_per_s": processed / (t1 - t0),
-                     "ms_per_step": 1e3 * (t1 - t0) / steps, "kernel_ms_per_launch": kern_ms / max(launches, 1),
-                     "processed_per_launch": (s2["processed"] - s1["processed"]) / max(launches, 1),
+                     "ms_per_step": 1e3 * (t1 - t0) / steps, "launch_ms_median": float(np.median(ms)),
+                     "launch_ms_mean": float(np.mean(ms)), "launches_timed": len(ms),
+                     "processed_per_launch": (s2["processed"] - s1["processed"]) / max(len(ms), 1),
                      "steps_per_launch": w.refresh, "settled": s1["settled"], "deferred": s1["deferred"] - s0["deferred"],
                      "precision": prec}
         log(f"[bench] chamber {prec}: {out[prec]}")
@@ -211,6 +287,50 @@ def chamber_run(steps: int):
     out["particles"] = len(w.pos)
     out["tets"] = int(w.mesh.n_tets)
     out["nodes"] = int(w.mesh.n_nodes)
+    return out
+
+
+def cpp_loop_run(w, steps: int, warmup: int):
+    """The drop-in path itself: the C++ ParticleDynamicsProcessor::process and
+    NodeChargeDensityProcessor::gather driven by shim_check the way the
+    reference's loop calls them (gather, then process, every step;
+    ModelingMainDriver.cpp:146-204) on the full-size large box, with the
+    density map filled every step.  Once with the default movement map
+    (200,000 particles recorded every step, ParticleMovementTracker.hpp:43)
+    and once without.  Wall time of `steps` steps after `warmup`."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "start_amd", "shim_check")
+    types = ["O2", "Ar", "Ne", "He", "Ti", "Al", "Sn", "W", "Au", "Cu", "Ni", "Ag"]
+    out = {}
+    with tempfile.TemporaryDirectory(prefix="cpp_loop_", dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        w.mesh.nodes.astype(np.float64).tofile(os.path.join(d, "nodes.f64"))
+        w.mesh.tets.astype(np.int32).tofile(os.path.join(d, "tets.i32"))
+        w.mesh.tris.astype(np.int32).tofile(os.path.join(d, "tris.i32"))
+        w.phi.astype(np.float64).tofile(os.path.join(d, "phi.f64"))
+        w.pos.astype(np.float64).tofile(os.path.join(d, "pos.f64"))
+        w.vel.astype(np.float64).tofile(os.path.join(d, "vel.f64"))
+        np.full(len(w.pos), types.index("Ar"), dtype=np.int32).tofile(os.path.join(d, "type.i32"))
+        np.arange(len(w.pos), dtype=np.uint64).tofile(os.path.join(d, "id.u64"))
+        sp, g = w.species[0], w.gas
+        for label, movements in (("no_movement_map", 0), ("movement_map_200k", 200_000)):
+            cfg = {"dt": repr(w.dt), "steps": warmup + steps, "model": g.model.lower(), "gas": "Ar",
+                   "pressure": 1.0, "temperature": 300.0, "seed": w.seed, "precision": "fp64",
+                   "density": "barycentric", "diameter_Ar": repr(sp.diameter),
+                   "charge_Ar": repr(sp.charge / 1.602176565e-19), "movements": movements,
+                   "timing_warmup": warmup}
+            with open(os.path.join(d, "config.txt"), "w") as f:
+                f.write("".join(f"{k} {v}\n" for k, v in cfg.items()))
+            r = subprocess.run([exe, d], capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError("shim_check failed: " + r.stdout + r.stderr)
+            t = dict(line.split() for line in open(os.path.join(d, "out_timing.txt")) if line.strip())
+            ts, loop = int(t["timed_steps"]), float(t["loop_s"])
+            out[label] = {"steps": ts, "ms_per_step": round(1e3 * loop / max(ts, 1), 4),
+                          "gather_ms_per_step": round(1e3 * float(t["gather_s"]) / max(ts, 1), 4),
+                          "process_ms_per_step": round(1e3 * float(t["process_s"]) / max(ts, 1), 4),
+                          "particle_steps_per_s": round(int(t["particle_steps"]) / loop, 1)}
+            log(f"[bench] C++ loop {label}: {out[label]}")
     return out
 
 
@@ -248,17 +368,41 @@ def adjoint_run(reps: int = 3):
                     "lambda upload; best of %d" % reps}
 
 
-def roofline(r, mode):
-    prec = MODES[mode][0] if mode in MODES else r["precision"]
-    B = BYTES_PER_PS[prec]
-    dur = r["kernel_ms_per_launch"] * 1e-3
-    achieved = B * r["processed_per_launch"] / dur / 1e9 if dur > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(mode),
-            "kernel": f"step_main<{'double' if prec == 'fp64' else 'float'}>",
-            "bytes_per_particle_step": B, "steps_per_launch": r["steps_per_launch"],
-            "particle_steps_per_launch": round(r["processed_per_launch"]),
-            "launch_ms": round(r["kernel_ms_per_launch"], 4)}
+def roofline(r, key: str):
+    """Roofline of the streaming kernel in one run: HBM for single-step
+    launches (algorithmic bytes over the median launch duration), VALU issue
+    for fused launches (counted VALU instructions per particle-step, from the
+    committed counter pass of the same kernel and launch shape).  `key` names
+    that counter pass in profiles/traffic.json."""
+    prec, K = r["precision"], r["steps_per_launch"]
+    dur = r["launch_ms_median"] * 1e-3
+    ps = r["processed_per_launch"]
+    prof = load_profile(key)
+    hbm_ps = prof.get("hbm_bytes_per_particle_step") if prof else None
+    common = {"kernel": f"step_main<{'double' if prec == 'fp64' else 'float'}>", "steps_per_launch": K,
+              "particle_steps_per_launch": round(ps), "launch_ms": round(r["launch_ms_median"], 4),
+              "launch_ms_is": f"median of {r['launches_timed']} launches (HIP events)",
+              "traffic": round(hbm_ps * ps) if hbm_ps is not None else None,
+              "hbm_bytes_per_particle_step_measured": hbm_ps}
+    if K == 1:
+        B = BYTES_PER_PS[prec]
+        achieved = B * ps / dur / 1e9 if dur > 0 else 0.0
+        out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_particle_step": B, **common}
+    else:
+        valu_ps = prof.get("valu_per_particle_step") if prof else None
+        if valu_ps is None:
+            return {"bound": "valu", "achieved": None, "peak": VALU_PEAK, "unit": "wave-instructions/s",
+                    "frac": None, "note": "no VALU counter pass for this shape in profiles/traffic.json", **common}
+        achieved = valu_ps * ps / dur if dur > 0 else 0.0
+        out = {"bound": "valu", "achieved": round(achieved, 1), "peak": VALU_PEAK, "unit": "wave-instructions/s",
+               "frac": round(achieved / VALU_PEAK, 4), "valu_per_particle_step": valu_ps, **common}
+        if hbm_ps is not None and dur > 0:
+            out["hbm_frac_measured"] = round(hbm_ps * ps / dur / 1e9 / HBM_PEAK_GBS, 4)
+    if out["frac"] is not None and out["frac"] > 1.0:
+        out["valid"] = False
+        log(f"[bench] INVALID roofline for {key}: frac {out['frac']} > 1")
+    return out
 
 
 def cpu_baseline(w, budget_s: float = 12.0, max_steps: int = 200):
@@ -306,14 +450,14 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def load_traffic(mode: str):
-    """HBM bytes per launch of step_main, from the committed rocprofv3 PMC summary."""
+def load_profile(key: str):
+    """Per-particle-step counters of one kernel and launch shape from the
+    committed rocprofv3 counter passes (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        d = json.load(open(path))
-        return d.get(mode, {}).get("hbm_bytes_per_launch")
+        return json.load(open(path)).get(key)
     except Exception:
         return None
 
@@ -332,6 +476,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--chamber-steps", type=int, default=1000,
                     help="steps of the 16M-particle chamber leg (mixed, then FP64); 0 skips it")
+    ap.add_argument("--cpp-loop-steps", type=int, default=200,
+                    help="steps of the C++ gather/process loop leg (shim_check, full large box); 0 skips it")
+    ap.add_argument("--accuracy-steps", type=int, default=400,
+                    help="steps of the mixed-vs-FP64 per-result error leg; 0 skips it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES, 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -362,6 +510,14 @@ def main():
         steps = args.steps if m == HEADLINE else args.mode_steps
         results[m] = gpu_run(w, m, steps, args.warmup)
         log(f"[bench] {m}: {results[m]}")
+    acc = None
+    if args.accuracy_steps > 0:
+        t = time.time()
+        acc = accuracy_run(w, args.accuracy_steps)
+        log(f"[bench] accuracy ({time.time() - t:.1f}s): {acc}")
+    cpp = None
+    if args.cpp_loop_steps > 0:
+        cpp = cpp_loop_run(w, args.cpp_loop_steps, 20)
     sweep = None
     if args.sweep_steps > 0:
         sweep = sweep_run(args.sweep_steps, args.warmup)
@@ -408,6 +564,15 @@ def main():
                       "precision": v["precision"], "steps_per_launch": v["steps_per_launch"],
                       "roofline": roofline(v, m)} for m, v in results.items()},
     }
+    if acc:
+        out["mixed_vs_fp64"] = acc
+    if cpp:
+        out["cpp_loop"] = {"workload": "large_box at full size through ParticleDynamicsProcessor::process and "
+                                       "NodeChargeDensityProcessor::gather (shim_check), FP64, density map every step",
+                           **cpp}
+        if "fp64_per_step" in results:
+            out["cpp_loop"]["vs_fp64_per_step"] = {
+                k: round(v["particle_steps_per_s"] / results["fp64_per_step"]["ps_per_s"], 3) for k, v in cpp.items()}
     if cpu:
         out["speedup_vs_cpu_baseline"] = round(r["ps_per_s"] / cpu["value"], 1)
     if sweep:
@@ -444,6 +609,18 @@ def main():
             "substrate_l1_mixed_vs_fp64": chamber["substrate_l1_mixed_vs_fp64"],
         }
     print(json.dumps(out), flush=True)
+    bad = []
+
+    def walk(o, path):
+        if isinstance(o, dict):
+            if o.get("valid") is False:
+                bad.append(path)
+            for k, v in o.items():
+                walk(v, f"{path}.{k}")
+    walk(out, "")
+    if bad:
+        log(f"[bench] roofline fraction above 1 (invalid measurement) at {bad}")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -349,6 +349,12 @@ int start_read_particles(start_ctx* ctx, double* x, double* y, double* z, double
 int start_read_tri_counts(start_ctx* ctx, uint64_t* counts /* n_tris */);
 /* Settled ids (ParticlesIDSet, PICTypes.hpp:41) in load order; *n in = capacity, out = count. */
 int start_read_settled(start_ctx* ctx, uint64_t* ids, int64_t* n);
+/* The ids of the particles that settled after the first `first` to settle
+ * (ParticlesIDSet inserts, ParticleSettler.cpp:7-26), in the order they
+ * settled (within one step: device order): what a caller that has inserted
+ * the first `first` still needs, with no read-back of the whole population.
+ * ids NULL: *n = the number of such particles; otherwise *n in = capacity. */
+int start_read_settled_since(start_ctx* ctx, int64_t first, uint64_t* ids, int64_t* n);
 int start_read_stats(start_ctx* ctx, start_stats* out);
 int start_synchronize(start_ctx* ctx);
 
@@ -382,6 +388,10 @@ int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign);
  * stream; returns the summed kernel milliseconds since the last reset. */
 int start_timing_enable(start_ctx* ctx, int32_t on);
 int start_timing_read(start_ctx* ctx, double* step_kernel_ms, uint64_t* launches);
+/* The same, launch by launch: with ms NULL, *n = the number of launches timed;
+ * otherwise (*n in = capacity) ms[k] = the streaming kernel's milliseconds in
+ * launch k, and the timings are reset. */
+int start_timing_read_each(start_ctx* ctx, double* ms, int64_t* n);
 
 #ifdef __cplusplus
 }

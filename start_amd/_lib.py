@@ -28,7 +28,7 @@ EXPORTS = [
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
     "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
     "start_read_tet_field", "start_generate", "start_track_movements", "start_read_movements",
-    "start_clear_movements", "start_locate_noalloc", "start_append_particles", "start_mesh_maps",
+    "start_clear_movements", "start_locate_noalloc", "start_append_particles", "start_mesh_maps", "start_timing_read_each", "start_read_settled_since",
 ]
 NOALLOC_LOCATE_MAX = 131072
 SOURCE = {"point": 0, "surface": 1}
@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
     L.start_read_particles.argtypes = [vp] + [P(d)] * 6 + [P(i32)] * 3
     L.start_read_tri_counts.argtypes = [vp, P(u64)]
     L.start_read_settled.argtypes = [vp, P(u64), P(i64)]
+    L.start_read_settled_since.argtypes = [vp, i64, P(u64), P(i64)]
     L.start_read_stats.argtypes = [vp, P(Stats)]
     L.start_synchronize.argtypes = [vp]
     L.start_locate.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
@@ -123,6 +124,7 @@ def lib() -> C.CDLL:
     L.start_mesh_maps.argtypes = [P(MeshDesc), P(i32), P(d), P(d), P(d), P(C.c_float), P(d)]
     L.start_timing_enable.argtypes = [vp, i32]
     L.start_timing_read.argtypes = [vp, P(d), P(u64)]
+    L.start_timing_read_each.argtypes = [vp, P(d), P(i64)]
     L.start_node_charge_density.argtypes = [vp, i32, P(d), P(i32)]
     L.start_set_gas.argtypes = [vp, P(GasDesc), P(d)]
     L.start_set_time_step.argtypes = [vp, d]

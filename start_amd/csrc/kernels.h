@@ -127,6 +127,8 @@ struct StepParams {
     int32_t work_slot;        // slot of this launch's work-list, step and tile counters
     int32_t nsteps;           // steps fused in this launch (charge accumulates over them)
     SettleRec* settle;        // null, or one record per load index
+    uint32_t* settle_log;     // load index of each particle in the order it settled
+                              // (slot = the settled count before it; one per particle)
     // movement records (ParticleMovementTracker.cpp:4-16): null, or the slot
     // of each load index (-1: not tracked), points recorded per slot, and
     // trk_max absolute points per slot

@@ -168,6 +168,12 @@ struct start_ctx {
     // node charge density tables (built on first use)
     double *d_tet_vol = nullptr, *d_node_vol = nullptr, *d_tet_q = nullptr;
     int32_t *d_adj_start = nullptr, *d_adj_list = nullptr, *d_tet_n = nullptr;
+    // node density output: device buffers and pinned host staging, kept
+    // between calls (the step loop asks for it every step)
+    double* d_rho = nullptr;
+    int32_t* d_written = nullptr;
+    double* h_rho = nullptr;
+    int32_t* h_written = nullptr;
     // particles: n live rows in buffers of cap rows (appends grow cap geometrically)
     int64_t n = 0;
     int64_t cap = 0;
@@ -182,6 +188,8 @@ struct start_ctx {
     unsigned long long* d_cnt = nullptr;
     uint64_t step = 0;
     uint64_t last_sort = 0;
+    // load indices in settle order (cap entries: a particle settles once)
+    uint32_t* d_settle_log = nullptr;
     // settle records for the adjoint (by load index), when recording
     SettleRec* d_settle = nullptr;
     bool record = false;
@@ -354,7 +362,9 @@ void free_particles(start_ctx* c)
     free_pbuf(c->A);
     free_work_buffers(c);
     if (c->d_settle) (void)hipFree(c->d_settle);
+    if (c->d_settle_log) (void)hipFree(c->d_settle_log);
     c->d_settle = nullptr;
+    c->d_settle_log = nullptr;
     c->n = 0;
     c->cap = 0;
 }
@@ -554,6 +564,7 @@ StepParams step_params(const start_ctx* c)
     S.n_active = c->d_nactive;
     S.charge_len = c->charge_len();
     S.settle = c->record ? c->d_settle : nullptr;
+    S.settle_log = c->d_settle_log;
     S.trk_slot = c->d_trk_slot;
     S.trk_count = c->d_trk_count;
     S.trk_xyz = c->d_trk_xyz;
@@ -754,6 +765,7 @@ int begin_population(start_ctx* c, const std::vector<RunK>& rk, int64_t n, const
     if (rc != START_OK) return rc;
     hipError_t he = alloc_pbuf(c->A, n, c->es);
     if (he == hipSuccess) he = alloc_work_buffers(c, n);
+    if (he == hipSuccess) he = dalloc(&c->d_settle_log, n);
     if (he != hipSuccess) {
         free_particles(c);
         return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
@@ -799,8 +811,13 @@ int grow_particles(start_ctx* c, int64_t need)
     PBuf A2;
     SettleRec* s2 = nullptr;
     int32_t* slot2 = nullptr;
+    uint32_t* log2 = nullptr;
     hipError_t he = alloc_pbuf(A2, cap, c->es);
     if (he == hipSuccess && c->n > 0) he = copy_pbuf(A2, c->A, c->n, c->es, c->stream);
+    if (he == hipSuccess) he = dalloc(&log2, cap);
+    if (he == hipSuccess && c->n > 0 && c->d_settle_log)  // entries past the settled count are never read
+        he = hipMemcpyAsync(log2, c->d_settle_log, sizeof(uint32_t) * (size_t)c->n, hipMemcpyDeviceToDevice,
+                            c->stream);
     if (he == hipSuccess && c->d_settle) {  // step = -1: not settled
         he = dalloc(&s2, cap);
         if (he == hipSuccess) he = hipMemsetAsync(s2, 0xff, sizeof(SettleRec) * (size_t)cap, c->stream);
@@ -820,11 +837,14 @@ int grow_particles(start_ctx* c, int64_t need)
         free_pbuf(A2);
         if (s2) (void)hipFree(s2);
         if (slot2) (void)hipFree(slot2);
+        if (log2) (void)hipFree(log2);
         return fail(c, he == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
                     std::string("particle buffers (growing): ") + hipGetErrorString(he));
     }
     free_pbuf(c->A);
     c->A = A2;
+    if (c->d_settle_log) (void)hipFree(c->d_settle_log);
+    c->d_settle_log = log2;
     if (s2) {
         (void)hipFree(c->d_settle);
         c->d_settle = s2;
@@ -1130,10 +1150,13 @@ void start_destroy(start_ctx* c)
                     c->d_E1_32,     c->d_tnode,     c->d_tnbr,        c->d_tris,      c->d_tri_deg,  c->d_tgs,
                     c->d_tgl,       c->d_sgs,       c->d_sgl,         c->d_tri_count, c->d_node_charge,
                     c->d_species,   c->d_cnt,       c->d_nactive,     c->d_runs,      c->d_tet_vol,  c->d_node_vol,
-                    c->d_tet_q,     c->d_adj_start, c->d_adj_list,    c->d_tet_n,     c->d_phi};
+                    c->d_tet_q,     c->d_adj_start, c->d_adj_list,    c->d_tet_n,     c->d_phi,
+                    c->d_rho,       c->d_written};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_phi) (void)hipHostFree(c->h_phi);
+    if (c->h_rho) (void)hipHostFree(c->h_rho);
+    if (c->h_written) (void)hipHostFree(c->h_written);
     if (c->phi_copied) (void)hipEventDestroy(c->phi_copied);
     for (auto& e : c->events) {
         (void)hipEventDestroy(e.a0);
@@ -1800,33 +1823,36 @@ int start_node_charge_density(start_ctx* c, int32_t mode, double* rho, int32_t* 
     int rc = ensure_density_tables(c);
     if (rc != START_OK) return rc;
     const int64_t M = c->hm.n_nodes, T = c->hm.n_tets;
-    double* d_rho = nullptr;
-    int32_t* d_wr = nullptr;
-    hipError_t e = dalloc(&d_rho, M);
-    if (e == hipSuccess && written) e = dalloc(&d_wr, M);
-    if (e == hipSuccess && mode == START_DENSITY_TET_AVERAGE) {
-        e = hipMemsetAsync(c->d_tet_q, 0, sizeof(double) * T, c->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(c->d_tet_n, 0, sizeof(int32_t) * T, c->stream);
+    if (!c->d_rho) {
+        hipError_t e = dalloc(&c->d_rho, M);
+        if (e == hipSuccess) e = dalloc(&c->d_written, M);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_rho, sizeof(double) * (size_t)M, hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void**)&c->h_written, sizeof(int32_t) * (size_t)M, hipHostMallocDefault);
+        if (e != hipSuccess)
+            return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                        std::string("node charge density buffers: ") + hipGetErrorString(e));
     }
-    if (e == hipSuccess) {
-        const double* q = c->d_node_charge + (size_t)c->latest * M;
-        if (c->mixed)
-            launch_node_density<float>(c->stream, c->view<float>(), c->d_species, mode, q, c->d_node_vol,
-                                       c->d_adj_start, c->d_adj_list, c->d_tet_vol, c->d_tet_q, c->d_tet_n, M, d_rho,
-                                       d_wr);
-        else
-            launch_node_density<double>(c->stream, c->view<double>(), c->d_species, mode, q, c->d_node_vol,
-                                        c->d_adj_start, c->d_adj_list, c->d_tet_vol, c->d_tet_q, c->d_tet_n, M, d_rho,
-                                        d_wr);
-        e = hipGetLastError();
+    if (mode == START_DENSITY_TET_AVERAGE) {
+        HIPCHECK(c, hipMemsetAsync(c->d_tet_q, 0, sizeof(double) * T, c->stream));
+        HIPCHECK(c, hipMemsetAsync(c->d_tet_n, 0, sizeof(int32_t) * T, c->stream));
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(rho, d_rho, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && written)
-        e = hipMemcpyAsync(written, d_wr, sizeof(int32_t) * M, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (d_rho) (void)hipFree(d_rho);
-    if (d_wr) (void)hipFree(d_wr);
-    if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("node charge density: ") + hipGetErrorString(e));
+    const double* q = c->d_node_charge + (size_t)c->latest * M;
+    if (c->mixed)
+        launch_node_density<float>(c->stream, c->view<float>(), c->d_species, mode, q, c->d_node_vol,
+                                   c->d_adj_start, c->d_adj_list, c->d_tet_vol, c->d_tet_q, c->d_tet_n, M, c->d_rho,
+                                   c->d_written);
+    else
+        launch_node_density<double>(c->stream, c->view<double>(), c->d_species, mode, q, c->d_node_vol,
+                                    c->d_adj_start, c->d_adj_list, c->d_tet_vol, c->d_tet_q, c->d_tet_n, M, c->d_rho,
+                                    c->d_written);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipMemcpyAsync(c->h_rho, c->d_rho, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream));
+    if (written)
+        HIPCHECK(c, hipMemcpyAsync(c->h_written, c->d_written, sizeof(int32_t) * M, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(rho, c->h_rho, sizeof(double) * (size_t)M);
+    if (written) std::memcpy(written, c->h_written, sizeof(int32_t) * (size_t)M);
     return START_OK;
 }
 
@@ -1933,6 +1959,28 @@ int start_read_settled(start_ctx* c, uint64_t* ids, int64_t* n)
             ++k;
         }
     *n = k;
+    return START_OK;
+}
+
+int start_read_settled_since(start_ctx* c, int64_t first, uint64_t* ids, int64_t* n)
+{
+    if (!c || !n || first < 0) return fail(c, START_ERR_ARG, "bad argument");
+    unsigned long long settled = 0;
+    HIPCHECK(c, hipMemcpyAsync(&settled, c->d_cnt + kCntSettled, sizeof(settled), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const int64_t total = (int64_t)settled, m = std::max<int64_t>(0, total - first);
+    if (!ids) {
+        *n = m;
+        return START_OK;
+    }
+    if (*n < m) return fail(c, START_ERR_ARG, "capacity below the number of newly settled particles");
+    *n = m;
+    if (m == 0) return START_OK;
+    std::vector<uint32_t> li((size_t)m);
+    HIPCHECK(c, hipMemcpyAsync(li.data(), c->d_settle_log + first, sizeof(uint32_t) * (size_t)m,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int64_t k = 0; k < m; ++k) ids[k] = c->ids[li[(size_t)k]];
     return START_OK;
 }
 
@@ -2065,6 +2113,24 @@ int start_timing_enable(start_ctx* c, int32_t on)
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     c->timing = on != 0;
     c->ev_used = 0;
+    return START_OK;
+}
+
+int start_timing_read_each(start_ctx* c, double* ms, int64_t* n)
+{
+    if (!c || !n) return fail(c, START_ERR_ARG, "null argument");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const int64_t m = (int64_t)c->ev_used;
+    if (ms) {
+        if (*n < m) return fail(c, START_ERR_ARG, "capacity below the number of timed launches");
+        for (int64_t k = 0; k < m; ++k) {
+            float t = 0.0f;
+            HIPCHECK(c, hipEventElapsedTime(&t, c->events[(size_t)k].a0, c->events[(size_t)k].a1));
+            ms[k] = t;
+        }
+        c->ev_used = 0;
+    }
+    *n = m;
     return START_OK;
 }
 

@@ -323,6 +323,15 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_read_settled(self.ctx, _up(out), C.byref(n)))
         return out[: n.value]
 
+    def settled_ids_since(self, first: int) -> np.ndarray:
+        """Ids of the particles that settled after the first `first` to settle, in settle order."""
+        L = _lib.lib()
+        n = C.c_int64(0)
+        self._check(L.start_read_settled_since(self.ctx, int(first), None, C.byref(n)))
+        out = np.empty(max(n.value, 1), dtype=np.uint64)
+        self._check(L.start_read_settled_since(self.ctx, int(first), _up(out), C.byref(n)))
+        return out[: n.value]
+
     def stats(self) -> dict:
         s = _lib.Stats()
         self._check(_lib.lib().start_read_stats(self.ctx, C.byref(s)))
@@ -348,6 +357,15 @@ class ParticleInCellStep:
 
     def timing(self, on: bool):
         self._check(_lib.lib().start_timing_enable(self.ctx, int(on)))
+
+    def timing_read_each(self) -> np.ndarray:
+        """Milliseconds of the streaming kernel in each timed launch (resets)."""
+        L = _lib.lib()
+        n = C.c_int64(0)
+        self._check(L.start_timing_read_each(self.ctx, None, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.float64)
+        self._check(L.start_timing_read_each(self.ctx, _dp(out), C.byref(n)))
+        return out[: n.value]
 
     def timing_read(self):
         ms = C.c_double(0)

@@ -5,6 +5,7 @@
 #include <cctype>
 #include <cmath>
 #include <iostream>
+#include <iterator>
 #include <utility>
 
 namespace start_amd {
@@ -100,6 +101,12 @@ ParticleKernelDevice::ParticleKernelDevice(DeviceMeshInput mesh, DeviceKernelOpt
         for (size_t s = 0; s < S; ++s) m_mesh.triangleTags.push_back(s);
     if (m_mesh.nodeTags.size() != M || m_mesh.triangleTags.size() != S)
         throw std::invalid_argument("tag arrays do not match the mesh");
+    m_rowsByTag.resize(M);
+    for (size_t n = 0; n < M; ++n) m_rowsByTag[n] = n;
+    std::sort(m_rowsByTag.begin(), m_rowsByTag.end(),
+              [&](size_t a, size_t b) { return m_mesh.nodeTags[a] < m_mesh.nodeTags[b]; });
+    m_rho.resize(M);
+    m_written.resize(M);
 }
 
 ParticleKernelDevice::~ParticleKernelDevice()
@@ -257,10 +264,10 @@ void ParticleKernelDevice::step(ParticleVector& particles, ParticlesIDSet& settl
     m_charge_fresh = true;  // the step scatters the end-of-step charge
     start_stats s{};
     check(start_read_stats(m_ctx, &s), "step statistics");
-    if (s.settled != m_settled_seen) {
-        std::vector<uint64_t> ids((size_t)s.settled);
-        int64_t k = (int64_t)ids.size();
-        check(start_read_settled(m_ctx, ids.data(), &k), "settled ids");
+    if (s.settled > m_settled_seen) {  // only the particles that settled since the last step
+        int64_t k = (int64_t)(s.settled - m_settled_seen);
+        std::vector<uint64_t> ids((size_t)k);
+        check(start_read_settled_since(m_ctx, (int64_t)m_settled_seen, ids.data(), &k), "settled ids");
         settled.insert(ids.begin(), ids.begin() + k);
         m_settled_seen = s.settled;
     }
@@ -288,12 +295,22 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
         check(start_gather_charge(m_ctx), "charge scatter");
         m_charge_fresh = true;
     }
-    size_t M = m_mesh.nodes.size() / 3;
-    std::vector<double> rho(M);
-    std::vector<int32_t> wr(M);
-    check(start_node_charge_density(m_ctx, m_opt.densityMode, rho.data(), wr.data()), "node charge density");
-    for (size_t n = 0; n < M; ++n)
-        if (wr[n]) out[m_mesh.nodeTags[n]] = rho[n];
+    check(start_node_charge_density(m_ctx, m_opt.densityMode, m_rho.data(), m_written.data()), "node charge density");
+    // One ordered pass: the driver keeps one map for the whole run and never
+    // clears it (ModelingMainDriver.cpp:122), so after the first step every
+    // key is already there and is only updated; keys of other nodes are kept.
+    auto it = out.begin();
+    for (size_t r : m_rowsByTag) {
+        if (!m_written[r]) continue;
+        const size_t tag = m_mesh.nodeTags[r];
+        while (it != out.end() && it->first < tag) ++it;
+        if (it != out.end() && it->first == tag) {
+            it->second = m_rho[r];
+            ++it;
+        } else {
+            it = std::next(out.emplace_hint(it, tag, m_rho[r]));
+        }
+    }
 }
 
 void ParticleKernelDevice::setNodePotential(const std::map<size_t, double>& potentialByTag)

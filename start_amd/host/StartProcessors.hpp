@@ -91,7 +91,11 @@ struct DeviceKernelOptions {
     uint64_t seed = 1;
     uint32_t run = 0;
     double bboxExpansion = 1.01;       // Constants.hpp:55
-    bool writeBackEachStep = true;     // update the ParticleVector after every process()
+    // Update the ParticleVector after every process().  Off by default: the
+    // reference's loop never reads particle state between steps (gather and
+    // process read the device) nor after it (ModelingMainDriver.cpp:79-91
+    // writes triangle counts and movements); syncParticles() reads it on demand.
+    bool writeBackEachStep = false;
     // particleMovementMap of process(): the first movementMaxParticles moving
     // particles are recorded (ParticleMovementTracker.hpp:43); 0 records none
     size_t movementMaxParticles = 200000;
@@ -155,6 +159,9 @@ private:
 
     DeviceMeshInput m_mesh;
     DeviceKernelOptions m_opt;
+    std::vector<size_t> m_rowsByTag;     // node rows in ascending tag order
+    std::vector<double> m_rho;           // density read-back, by node row
+    std::vector<int32_t> m_written;
     start_ctx* m_ctx = nullptr;
     bool m_configured = false;
     double m_dt = 0.0;

@@ -16,13 +16,20 @@
 //                 spawn_type.i32 (K) spawn_id.u64 (K), in order, are inserted
 //                 at the end of the ParticleVector (continuous sputtering,
 //                 ModelingMainDriver.cpp:37-71, :141-142); optional shrink_at s:
-//                 the last particle is removed before step s (an error)
+//                 the last particle is removed before step s (an error);
+//                 optional movements N (particles tracked in the movement map,
+//                 200,000 by default as ParticleMovementTracker.hpp:43; 0: none),
+//                 timing_warmup W (steps before the timed window)
 // Outputs in <dir>:
 //   out_pos.f64 out_vel.f64  particle state after the loop (ParticleVector order)
 //   out_settled.u64           settled ids, ascending
 //   out_tri_counts.u64        TriangleCell::count by triangle tag (row order)
 //   out_density.f64 out_density_written.i32   the last gather's map, by node row
 //   out_summary.txt           steps run, stop flag
+//   out_timing.txt            wall seconds of the loop after timing_warmup
+//                             steps, of its gather and process calls, and
+//                             the particle-steps processed in that window
+#include <chrono>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -77,7 +84,7 @@ int main(int argc, char** argv)
         opt.precision = cfg.at("precision") == "mixed" ? START_PREC_MIXED : START_PREC_FP64;
         opt.densityMode = cfg.at("density") == "tet_average" ? START_DENSITY_TET_AVERAGE : START_DENSITY_BARYCENTRIC;
         opt.emPush = cfg.count("em_push") ? std::stoi(cfg.at("em_push")) != 0 : true;
-        opt.writeBackEachStep = false;
+        if (cfg.count("movements")) opt.movementMaxParticles = (size_t)std::stoull(cfg.at("movements"));
         for (const auto& [k, v] : cfg) {
             if (k.rfind("diameter_", 0) == 0) opt.diameterOverride[typeFromName(k.substr(9))] = std::stod(v);
             if (k.rfind("charge_", 0) == 0) opt.chargeOverride[typeFromName(k.substr(7))] = std::stod(v) * 1.602176565e-19;
@@ -122,19 +129,44 @@ int main(int argc, char** argv)
         StopSubject stop;
         std::shared_mutex sh;
         std::mutex mx;
+        const int warm = cfg.count("timing_warmup") ? std::stoi(cfg.at("timing_warmup")) : 0;
+        using clk = std::chrono::steady_clock;
+        double t_gather = 0.0, t_process = 0.0;
+        clk::time_point t_begin = clk::now();
+        uint64_t processed0 = 0;
+        auto processed_now = [&]() {
+            start_stats st{};
+            if (!device.context() || start_read_stats(device.context(), &st) != START_OK) return (uint64_t)0;
+            return st.processed;
+        };
         int k = 0;
         double t = 0.0;
         for (; k < steps && !stop.stopRequested(); ++k, t += dt) {
+            if (k == warm) {
+                processed0 = processed_now();
+                t_begin = clk::now();
+                t_gather = t_process = 0.0;
+            }
             if (spawn_every > 0 && k > 0 && k % spawn_every == 0)
                 for (size_t j = 0; j < spawn_count && spawned < sid.size(); ++j, ++spawned)
                     particles.emplace_back((size_t)sid[spawned], (ParticleType)stype[spawned], spos[3 * spawned],
                                            spos[3 * spawned + 1], spos[3 * spawned + 2], svel[3 * spawned],
                                            svel[3 * spawned + 1], svel[3 * spawned + 2]);
             if (k == shrink_at) particles.pop_back();
+            const clk::time_point a = clk::now();
             NodeChargeDensityProcessor::gather(t, dir + "/config.txt", device, particles, settled, tracker, density);
+            const clk::time_point b = clk::now();
             ParticleDynamicsProcessor::process(dir + "/config.txt", particles, t, dt, 16, device, tracker, settled,
                                                sh, mx, movements, stop, cfg.at("model"), cfg.at("gas"), conc);
+            const clk::time_point e = clk::now();
+            t_gather += std::chrono::duration<double>(b - a).count();
+            t_process += std::chrono::duration<double>(e - b).count();
         }
+        const double t_loop = std::chrono::duration<double>(clk::now() - t_begin).count();
+        const uint64_t processed1 = processed_now();
+        std::ofstream(dir + "/out_timing.txt") << "timed_steps " << std::max(0, k - warm) << "\nloop_s " << t_loop
+                                               << "\ngather_s " << t_gather << "\nprocess_s " << t_process
+                                               << "\nparticle_steps " << (processed1 - processed0) << "\n";
         density.clear();
         NodeChargeDensityProcessor::gather(t, dir + "/config.txt", device, particles, settled, tracker, density);
         device.syncParticles(particles);

@@ -69,6 +69,10 @@ struct DevMesh {
     const double* E1;      // 3T, field basis 1 (null: one basis)
     const float* E1_32;    // 4T
     const float* geo32;    // 12T (mixed)
+    // push increments h = (((E*q)/m)*dt)/2 of field basis 0 per charged
+    // species slot and tet: FP64 3 per tet, slot-major; mixed float4 per tet
+    const double* H;
+    const float* H32;
     const int4* tnode;     // T
     const int4* tnbr;      // T
     const int32_t* tris;   // 3S
@@ -111,6 +115,8 @@ struct StepLaunch {
     int batch;       // 1: particles of several runs (per-lane run parameters)
     int walk;        // 1: certified multi-tet walk on the fast path (block 256 only)
 };
+
+constexpr int kMaxPushSlots = 4;
 
 struct StepParams {
     double dt;

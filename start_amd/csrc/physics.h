@@ -29,6 +29,11 @@ struct SpeciesK {
     double mass_cp;    // p_mass/(t_mass+p_mass)               HSModel.cpp:26
     double mass_ct;    // t_mass/(t_mass+p_mass)               HSModel.cpp:27
     double inv_alpha;  // 1./alpha                             VSSModel.cpp:32
+    // Push of a single simulation: >= 0 the slot of this species' table of
+    // push increments per tet; -1 computed in the kernel; -2 charge 0 (the
+    // increment is E*q, a signed zero)
+    int32_t push_slot;
+    int32_t pad;
 };
 
 __host__ __device__ inline void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,

@@ -45,6 +45,8 @@ void launch_track_eligible(hipStream_t s, const DevMesh& M, const DevParticles<R
                            uint8_t* out);
 void launch_field(hipStream_t s, const DevMesh& M, const double* phi, double* E, float* E32);
 void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
+void launch_push_table(hipStream_t s, const double* E, const float* E32, int64_t nt, const double* q,
+                       const double* m, int32_t n_slots, double dt, double* H, float* H32);
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
 void launch_abs_positions(hipStream_t s, const DevMesh& M, const float* lx, const float* ly, const float* lz,
                           const int32_t* tet, int64_t n, double* x, double* y, double* z);
@@ -150,6 +152,13 @@ struct start_ctx {
     double* h_phi = nullptr;
     hipEvent_t phi_copied = nullptr;
     bool phi_pending = false;
+    // push increments per charged species slot and tet (single simulation),
+    // refreshed before the next step whenever the field or the time step changed
+    double* d_H = nullptr;
+    float* d_H32 = nullptr;
+    int32_t n_push = 0;
+    bool push_dirty = true;
+    std::vector<uint8_t> present;  // species of the loaded population
     int4* d_tnode = nullptr;
     int4* d_tnbr = nullptr;
     int32_t* d_tris = nullptr;
@@ -221,6 +230,8 @@ struct start_ctx {
         M.E1 = d_E1;
         M.E1_32 = d_E1_32;
         M.geo32 = d_geo32;
+        M.H = d_H;
+        M.H32 = d_H32;
         M.tnode = d_tnode;
         M.tnbr = d_tnbr;
         M.tris = d_tris;
@@ -406,6 +417,8 @@ SpeciesK species_constants(const start_species_desc& s, const start_gas_desc& g)
     k.mass_cp = p_mass / (t_mass + p_mass);
     k.mass_ct = t_mass / (t_mass + p_mass);
     k.inv_alpha = g.alpha != 0.0 ? 1. / g.alpha : 0.0;
+    k.push_slot = s.charge == 0.0 ? -2 : -1;
+    k.pad = 0;
     return k;
 }
 
@@ -527,6 +540,73 @@ int upload_runs(start_ctx* c)
     HIPCHECK(c, dalloc(&c->d_runs, c->runs.size()));
     HIPCHECK(c, hipMemcpy(c->d_runs, c->runs.data(), sizeof(RunK) * c->runs.size(), hipMemcpyHostToDevice));
     drop_graphs(c);  // captured launches hold the old table pointer
+    return START_OK;
+}
+
+// Table slots for the charged species of the population (up to
+// kMaxPushSlots; others push in the kernel), and the tables they need.
+int assign_push_slots(start_ctx* c)
+{
+    std::vector<int32_t> slot(c->species.size(), -1);
+    int32_t n = 0;
+    for (size_t s = 0; s < c->species.size(); ++s) {
+        if (c->species[s].charge == 0.0) slot[s] = -2;
+        else if (s < c->present.size() && c->present[s] && n < kMaxPushSlots) slot[s] = n++;
+    }
+    bool same = n == c->n_push;
+    for (size_t s = 0; s < c->species.size(); ++s) same = same && c->species[s].push_slot == slot[s];
+    if (same) return START_OK;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (size_t s = 0; s < c->species.size(); ++s) c->species[s].push_slot = slot[s];
+    if (c->d_H) (void)hipFree(c->d_H);
+    if (c->d_H32) (void)hipFree(c->d_H32);
+    c->d_H = nullptr;
+    c->d_H32 = nullptr;
+    c->n_push = n;
+    const int64_t T = c->hm.n_tets;
+    hipError_t e = hipSuccess;
+    if (n > 0) e = c->mixed ? dalloc(&c->d_H32, 4 * (int64_t)n * T) : dalloc(&c->d_H, 3 * (int64_t)n * T);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->d_species, c->species.data(), sizeof(SpeciesK) * c->species.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        for (auto& k : c->species) k.push_slot = k.charge == 0.0 ? -2 : -1;
+        c->n_push = 0;
+        (void)hipMemcpy(c->d_species, c->species.data(), sizeof(SpeciesK) * c->species.size(), hipMemcpyHostToDevice);
+        return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                    std::string("push tables: ") + hipGetErrorString(e));
+    }
+    c->push_dirty = true;
+    drop_graphs(c);  // captured launches hold the table pointers
+    return START_OK;
+}
+
+// Species of n new particles join the population's set.
+int mark_present(start_ctx* c, const int32_t* species, int64_t n)
+{
+    c->present.resize(c->species.size(), 0);
+    for (int64_t i = 0; i < n; ++i) c->present[(size_t)species[i]] = 1;
+    return assign_push_slots(c);
+}
+
+// The tables of the current field and time step, queued on the library's
+// stream before the next step.
+int refresh_push_tables(start_ctx* c)
+{
+    if (!c->push_dirty) return START_OK;
+    if (c->n_push > 0) {
+        std::vector<double> q, m;
+        for (const auto& k : c->species)
+            if (k.push_slot >= 0) {
+                q.resize((size_t)std::max<int32_t>((int32_t)q.size(), k.push_slot + 1));
+                m.resize(q.size());
+                q[(size_t)k.push_slot] = k.charge;
+                m[(size_t)k.push_slot] = k.mass;
+            }
+        launch_push_table(c->stream, c->d_E, c->d_E32, c->hm.n_tets, q.data(), m.data(), c->n_push, c->prm.dt, c->d_H,
+                          c->d_H32);
+        HIPCHECK(c, hipGetLastError());
+    }
+    c->push_dirty = false;
     return START_OK;
 }
 
@@ -1151,7 +1231,7 @@ void start_destroy(start_ctx* c)
                     c->d_tgl,       c->d_sgs,       c->d_sgl,         c->d_tri_count, c->d_node_charge,
                     c->d_species,   c->d_cnt,       c->d_nactive,     c->d_runs,      c->d_tet_vol,  c->d_node_vol,
                     c->d_tet_q,     c->d_adj_start, c->d_adj_list,    c->d_tet_n,     c->d_phi,
-                    c->d_rho,       c->d_written};
+                    c->d_rho,       c->d_written,   c->d_H,           c->d_H32};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_phi) (void)hipHostFree(c->h_phi);
@@ -1203,6 +1283,9 @@ int start_load_batch(start_ctx* c, int32_t n_runs, const start_run_desc* runs, c
         }
     }
     int rc = begin_population(c, rk, n, ids);
+    if (rc != START_OK) return rc;
+    c->present.assign(c->species.size(), 0);
+    rc = mark_present(c, species, n);
     if (rc != START_OK || n == 0) return rc;
     int64_t nact = n;
     HIPCHECK(c, hipMemcpy(c->d_nactive, &nact, sizeof(int64_t), hipMemcpyHostToDevice));
@@ -1309,6 +1392,13 @@ int start_generate(start_ctx* c, int32_t n_sources, const start_source_desc* src
     rk[0].tri_base = 0;
     int rc = begin_population(c, rk, n, nullptr);
     if (rc != START_OK) return rc;
+    {
+        std::vector<int32_t> sps((size_t)n_sources);
+        for (int32_t s = 0; s < n_sources; ++s) sps[(size_t)s] = src[s].species;
+        c->present.assign(c->species.size(), 0);
+        rc = mark_present(c, sps.data(), n_sources);
+        if (rc != START_OK) return rc;
+    }
     rc = init_population_tail(c, n);
     if (rc != START_OK) return rc;
     double *d_cen = nullptr, *d_nor = nullptr, *t[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -1373,6 +1463,8 @@ int start_append_particles(start_ctx* c, int64_t m, const double* x, const doubl
     const int64_t n0 = c->n;
     int rc = grow_particles(c, n0 + m);
     if (rc != START_OK) return rc;
+    rc = mark_present(c, species, m);
+    if (rc != START_OK) return rc;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     // launches read their first step from the device counter, which steps
     // taken while the context held no particles did not advance
@@ -1419,7 +1511,10 @@ int start_set_gas(start_ctx* c, const start_gas_desc* g, const double* diameters
             sd[s].diameter = diameters[s];
         }
     std::vector<SpeciesK> k;
-    for (const auto& s : sd) k.push_back(species_constants(s, *g));
+    for (size_t s = 0; s < sd.size(); ++s) {
+        k.push_back(species_constants(sd[s], *g));
+        k.back().push_slot = c->species[s].push_slot;  // masses and charges are unchanged
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipMemcpy(c->d_species, k.data(), sizeof(SpeciesK) * k.size(), hipMemcpyHostToDevice));
     c->species_desc = sd;
@@ -1438,7 +1533,10 @@ int start_set_time_step(start_ctx* c, double dt)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");
     if (!(dt >= 0.0)) return fail(c, START_ERR_ARG, "time step must be >= 0");
-    if (dt != c->prm.dt) drop_graphs(c);
+    if (dt != c->prm.dt) {
+        drop_graphs(c);
+        c->push_dirty = true;
+    }
     c->prm.dt = dt;
     return START_OK;
 }
@@ -1458,6 +1556,7 @@ int start_set_tet_field_basis(start_ctx* c, int32_t which, const double* E)
     float* dE32 = which ? c->d_E1_32 : c->d_E32;
     HIPCHECK(c, hipMemcpyAsync(dE, E, sizeof(double) * 3 * T, hipMemcpyHostToDevice, c->stream));
     if (dE32) launch_e32(c->stream, dE, dE32, T);
+    if (which == 0) c->push_dirty = true;
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return START_OK;
@@ -1506,6 +1605,7 @@ int start_set_node_potential_basis(start_ctx* c, int32_t which, const double* ph
     HIPCHECK(c, hipEventRecord(c->phi_copied, c->stream));
     c->phi_pending = true;
     launch_field(c->stream, c->mesh_view(), c->d_phi, which ? c->d_E1 : c->d_E, which ? c->d_E1_32 : c->d_E32);
+    if (which == 0) c->push_dirty = true;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("field from potential: ") + hipGetErrorString(e));
     return START_OK;
@@ -1540,6 +1640,10 @@ int start_step(start_ctx* c, int32_t nsteps)
         return START_OK;
     }
     if (!c->d_wl || !c->B.x) return fail(c, START_ERR_STATE, "particle work buffers missing (a failed append)");
+    {
+        int rc = refresh_push_tables(c);
+        if (rc != START_OK) return rc;
+    }
     // Launches of steps_per_launch steps; before a launch, a sort if
     // sort_interval steps have passed since the last one.  A chunk is the run
     // of launches between two sorts.

@@ -440,6 +440,21 @@ __device__ inline void run_field(const DevMesh& M, const RunK* __restrict__ rk, 
         ez = ez + s1 * fz;
     }
 }
+// push increment of tet t for the species in table `slot`
+__device__ inline void load_push(const DevMesh& M, int slot, int32_t t, double& hx, double& hy, double& hz)
+{
+    const double* h = M.H + 3 * ((int64_t)slot * M.n_tets + t);
+    hx = h[0];
+    hy = h[1];
+    hz = h[2];
+}
+__device__ inline void load_push(const DevMesh& M, int slot, int32_t t, float& hx, float& hy, float& hz)
+{
+    const float4 h = reinterpret_cast<const float4*>(M.H32)[(int64_t)slot * M.n_tets + t];
+    hx = h.x;
+    hy = h.y;
+    hz = h.z;
+}
 __device__ inline void bary_at(const DevMesh& M, int32_t t, double x, double y, double z, double lam[4])
 {
     bary64(M.geo + 12 * (int64_t)t, x, y, z, lam);
@@ -923,9 +938,31 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
     R dt = (R)S.dt;
     // 4. push with the owning tet's field (ParticleDynamicsProcessor.cpp:179-180)
     if (S.em_push && s.t >= 0) {
-        R ex, ey, ez;
-        run_field<R, kBatch>(M, rk, s.t, ex, ey, ez);
-        push<R>(ex, ey, ez, (R)k.charge, (R)k.mass, dt, s.vx, s.vy, s.vz);
+        if (!kBatch && k.push_slot >= 0) {
+            // h = (((E*q)/m)*dt)/2 of this tet and species, evaluated per field
+            // and time step with the same operations (-0 where the push is
+            // skipped, which leaves v unchanged bit for bit)
+            R hx, hy, hz;
+            load_push(M, k.push_slot, s.t, hx, hy, hz);
+            s.vx = (s.vx + hx) + hx;
+            s.vy = (s.vy + hy) + hy;
+            s.vz = (s.vz + hz) + hz;
+        } else if (!kBatch && k.push_slot == -2) {
+            // charge 0: with m > 0 and dt > 0 the increment is the signed zero E*q
+            R ex, ey, ez;
+            load_E(M, s.t, ex, ey, ez);
+            if (dt != R(0) && !(ex == R(0) && ey == R(0) && ez == R(0))) {
+                const R q = (R)k.charge;
+                const R hx = ex * q, hy = ey * q, hz = ez * q;
+                s.vx = (s.vx + hx) + hx;
+                s.vy = (s.vy + hy) + hy;
+                s.vz = (s.vz + hz) + hz;
+            }
+        } else {
+            R ex, ey, ez;
+            run_field<R, kBatch>(M, rk, s.t, ex, ey, ez);
+            push<R>(ex, ey, ez, (R)k.charge, (R)k.mass, dt, s.vx, s.vy, s.vz);
+        }
     }
     // 6. move (Particle.cpp:157-165)
     R x1 = s.x + s.vx * dt, y1 = s.y + s.vy * dt, z1 = s.z + s.vz * dt;
@@ -1586,6 +1623,59 @@ __global__ void field_kernel(DevMesh M, const double* __restrict__ phi, double* 
         E32[4 * t + 2] = (float)ez;
         E32[4 * t + 3] = 0.0f;
     }
+}
+
+// Push increments h = (((E*q)/m)*dt)/2 of field basis 0 for each charged
+// species slot (push<R>'s arithmetic, FP64 from E and FP32 from the FP32 field),
+// -0 where push<R> returns early (dt == 0 or E == 0 exactly).
+struct PushSlots {
+    double q[kMaxPushSlots], m[kMaxPushSlots];
+    int32_t n;
+};
+template <typename R>
+__device__ inline R push_increment(R e, R q, R m, R dt)
+{
+    return (((e * q) / m) * dt) / R(2);
+}
+__global__ void push_table_kernel(const double* __restrict__ E, const float* __restrict__ E32, int64_t nt,
+                                  PushSlots ps, double dt, double* __restrict__ H, float* __restrict__ H32)
+{
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    for (int k = 0; k < ps.n; ++k) {
+        if (H) {
+            const double ex = E[3 * t], ey = E[3 * t + 1], ez = E[3 * t + 2];
+            const bool skip = dt == 0.0 || (ex == 0.0 && ey == 0.0 && ez == 0.0);
+            double* h = H + 3 * ((int64_t)k * nt + t);
+            h[0] = skip ? -0.0 : push_increment<double>(ex, ps.q[k], ps.m[k], dt);
+            h[1] = skip ? -0.0 : push_increment<double>(ey, ps.q[k], ps.m[k], dt);
+            h[2] = skip ? -0.0 : push_increment<double>(ez, ps.q[k], ps.m[k], dt);
+        }
+        if (H32) {
+            const float ex = E32[4 * t], ey = E32[4 * t + 1], ez = E32[4 * t + 2];
+            const float q = (float)ps.q[k], m = (float)ps.m[k], d = (float)dt;
+            const bool skip = d == 0.0f || (ex == 0.0f && ey == 0.0f && ez == 0.0f);
+            float4 h;
+            h.x = skip ? -0.0f : push_increment<float>(ex, q, m, d);
+            h.y = skip ? -0.0f : push_increment<float>(ey, q, m, d);
+            h.z = skip ? -0.0f : push_increment<float>(ez, q, m, d);
+            h.w = 0.0f;
+            reinterpret_cast<float4*>(H32)[(int64_t)k * nt + t] = h;
+        }
+    }
+}
+
+void launch_push_table(hipStream_t s, const double* E, const float* E32, int64_t nt, const double* q,
+                       const double* m, int32_t n_slots, double dt, double* H, float* H32)
+{
+    PushSlots ps{};
+    ps.n = n_slots;
+    for (int k = 0; k < n_slots && k < kMaxPushSlots; ++k) {
+        ps.q[k] = q[k];
+        ps.m[k] = m[k];
+    }
+    if (nt > 0 && n_slots > 0)
+        push_table_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(E, E32, nt, ps, dt, H, H32);
 }
 
 __global__ void e32_kernel(const double* __restrict__ E, float* __restrict__ E32, int64_t nt)

@@ -42,7 +42,8 @@ extern "C" {
 
 /* Arithmetic of the step. */
 #define START_PREC_FP64 0  /* FP64 throughout                                         */
-#define START_PREC_MIXED 1 /* FP32 push/collide/state, FP64 containment near faces and charge */
+#define START_PREC_MIXED 1 /* FP32 positions (stochastically rounded), tables and push increments; FP64 velocities,
+                             containment near faces, collision velocity and charge */
 
 /* Particle status as read back. */
 #define START_ACTIVE 0

@@ -1,7 +1,8 @@
 // Device data layout and kernel argument blocks of the particle step.
 //
 // Particles: structure of arrays in HBM.
-//   x y z vx vy vz   FP64 (FP64 mode) or FP32 (mixed mode)
+//   x y z            FP64 (FP64 mode) or FP32 in the owner's frame (mixed mode)
+//   vx vy vz         FP64
 //   tet              int32  owning tet at the start of the next step (-1: none)
 //   flags            int32  bit0 settled, bit1 position strictly inside `tet`,
 //                           bits 8..15 species index, bits 16..31 run index
@@ -90,7 +91,8 @@ struct DevMesh {
 
 template <typename R>
 struct DevParticles {
-    R *x, *y, *z, *vx, *vy, *vz;
+    R *x, *y, *z;
+    double *vx, *vy, *vz;
     int32_t* tet;
     int32_t* flags;
     int32_t* hit;

@@ -74,16 +74,47 @@ __host__ __device__ inline void variates(uint64_t seed, uint32_t run, uint32_t p
     ub = (double)b * two_m53;
 }
 
+// x rounded to FP32 stochastically: to the farther of the two neighbouring
+// floats with probability |x - nearest| / gap, resolved to 2^-10 by r10 (a
+// uniform 10-bit integer), so that the rounded value is x in expectation to
+// 2^-11 of a gap.  Repeated nearest rounding of x + d with a slowly changing
+// step d rounds the same way step after step and drifts linearly; this drifts
+// as a random walk.  (x is a local coordinate, far from the FP32 subnormals;
+// f == 0 with x != 0 gives a NaN g, and the comparison then keeps f.)
+__device__ inline float round_stochastic(double x, uint32_t r10)
+{
+    const float f = (float)x;
+    const double rem = x - (double)f;  // exact, |rem| <= gap / 2
+    const uint32_t fb = __float_as_uint(f);
+    const float g = __uint_as_float(((rem > 0.0) == (f > 0.0f)) ? fb + 1u : fb - 1u);  // next float toward x
+    const double gap = fabs((double)g - (double)f);
+    return ((double)r10 + 0.5) * gap < fabs(rem) * 1024.0 ? g : f;
+}
+
+// 32 hashed bits of (a, b, c): lowbias32 (C. Wellons' integer hash, public
+// domain) of a linear mix.  The mixed path draws its rounding bits from it.
+__device__ inline uint32_t hash3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t h = a * 0x9E3779B1u + b * 0x85EBCA77u + c;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
 // B = 0 push: v = (v + h) + h, h = (((E*q)/m)*dt)/2 per component; skipped when
-// dt == 0 or E == 0 exactly (Particle.cpp:199-207).
-template <typename R>
-__device__ inline void push(R ex, R ey, R ez, R q, R m, R dt, R& vx, R& vy, R& vz)
+// dt == 0 or E == 0 exactly (Particle.cpp:199-207).  h in R, the velocity
+// (always FP64 state) accumulated in V.
+template <typename R, typename V>
+__device__ inline void push(R ex, R ey, R ez, R q, R m, R dt, V& vx, V& vy, V& vz)
 {
     if (dt == R(0)) return;
     if (ex == R(0) && ey == R(0) && ez == R(0)) return;
-    R hx = (((ex * q) / m) * dt) / R(2);
-    R hy = (((ey * q) / m) * dt) / R(2);
-    R hz = (((ez * q) / m) * dt) / R(2);
+    V hx = (V)((((ex * q) / m) * dt) / R(2));
+    V hy = (V)((((ey * q) / m) * dt) / R(2));
+    V hz = (V)((((ez * q) / m) * dt) / R(2));
     vx = (vx + hx) + hx;
     vy = (vy + hy) + hy;
     vz = (vz + hz) + hz;
@@ -142,53 +173,37 @@ __device__ inline R collision_probability(int model, const CollideK<R>& k, R n, 
     return ((sigma * vmod) * n) * dt;
 }
 
-// Post-collision velocity for variates u2 (cos chi) and u3 (phi).
-// FP64 follows the reference expressions operation by operation.  FP32 uses
-// algebraically identical forms that avoid cancellation: sin(chi) as
-// sqrt((1-c)(1+c)), and the VSS rotation by beta = acos(pz/|p|),
-// gamma = atan2(py, px) written with the direction cosines of p.
-template <typename R>
-__device__ inline void scatter_velocity(int model, const CollideK<R>& k, R u2, R u3, R& vx, R& vy, R& vz)
+// Post-collision velocity for variates u2 (cos chi) and u3 (phi), in FP64 in
+// both precisions, following the reference expressions operation by operation
+// (HSModel.cpp:22-34, VHSModel.cpp:31-43, VSSModel.cpp:32-46).
+__device__ inline void scatter_velocity(int model, double mass_cp, double mass_ct, double inv_alpha, double u2,
+                                        double u3, double& vx, double& vy, double& vz)
 {
-    constexpr bool kF64 = sizeof(R) == 8;
-    R xi_cos;
-    if (model == 1) xi_cos = (u2 * (R(1) - (R(-1)))) + (R(-1));       // rng(-1, 1)
-    else if (model == 2) xi_cos = (u2 * (R(1) - R(0))) + R(0);         // rng()
-    else xi_cos = R(2) * m_pow<R>(u2, k.inv_alpha) - R(1);             // 2*pow(rng(),1/a)-1
-    R xi_sin = kF64 ? m_sqrt<R>(R(1) - xi_cos * xi_cos) : m_sqrt<R>((R(1) - xi_cos) * (R(1) + xi_cos));
-    R phi = (u3 * (R(2) * R(kPi) - R(0))) + R(0);                       // rng(0, 2*PI)
-    R sphi, cphi;
-    m_sincos<R>(phi, &sphi, &cphi);
-    R x = xi_sin * cphi, y = xi_sin * sphi, z = xi_cos;
-    R cmx = vx * k.mass_cp, cmy = vy * k.mass_cp, cmz = vz * k.mass_cp;
-    R px = vx * k.mass_ct, py = vy * k.mass_ct, pz = vz * k.mass_ct;
-    R mp = m_sqrt<R>(px * px + py * py + pz * pz);
-    R dx = x * mp, dy = y * mp, dz = z * mp;
+    double xi_cos;
+    if (model == 1) xi_cos = (u2 * (1.0 - (-1.0))) + (-1.0);        // rng(-1, 1)
+    else if (model == 2) xi_cos = (u2 * (1.0 - 0.0)) + 0.0;         // rng()
+    else xi_cos = 2.0 * m_pow<double>(u2, inv_alpha) - 1.0;          // 2*pow(rng(),1/a)-1
+    double xi_sin = m_sqrt<double>(1.0 - xi_cos * xi_cos);
+    double phi = (u3 * (2.0 * kPi - 0.0)) + 0.0;                      // rng(0, 2*PI)
+    double sphi, cphi;
+    m_sincos<double>(phi, &sphi, &cphi);
+    double x = xi_sin * cphi, y = xi_sin * sphi, z = xi_cos;
+    double cmx = vx * mass_cp, cmy = vy * mass_cp, cmz = vz * mass_cp;
+    double px = vx * mass_ct, py = vy * mass_ct, pz = vz * mass_ct;
+    double mp = m_sqrt<double>(px * px + py * py + pz * pz);
+    double dx = x * mp, dy = y * mp, dz = z * mp;
     if (model == 3) {
         // calcBetaGamma + rotation (GeometryVector.hpp:448-474, :66-90)
-        R sb, cb, sgm, cg;
-        if (kF64) {
-            R beta = m_acos<R>(pz / mp), gamma = m_atan2<R>(py, px);
-            m_sincos<R>(beta, &sb, &cb);
-            m_sincos<R>(gamma, &sgm, &cg);
-        } else {
-            R rho = m_sqrt<R>(px * px + py * py);
-            cb = pz / mp;
-            sb = rho / mp;
-            if (rho > R(0)) {
-                cg = px / rho;
-                sgm = py / rho;
-            } else {  // atan2(+-0, px): gamma = 0 or pi
-                cg = px < R(0) ? R(-1) : R(1);
-                sgm = R(0);
-            }
-        }
-        R tx = cb * dx + sb * dz;
-        R tz = -sb * dx + cb * dz;
+        double sb, cb, sgm, cg;
+        double beta = m_acos<double>(pz / mp), gamma = m_atan2<double>(py, px);
+        m_sincos<double>(beta, &sb, &cb);
+        m_sincos<double>(gamma, &sgm, &cg);
+        double tx = cb * dx + sb * dz;
+        double tz = -sb * dx + cb * dz;
         dx = tx;
         dz = tz;
         tx = cg * dx - sgm * dy;
-        R ty = sgm * dx + cg * dy;
+        double ty = sgm * dx + cg * dy;
         dx = tx;
         dy = ty;
     }

@@ -23,7 +23,7 @@ __global__ void make_keys_kernel(const int32_t* __restrict__ tet, const int32_t*
 }
 
 template <typename R>
-__global__ void permute_kernel(DevParticles<R> src, R* x, R* y, R* z, R* vx, R* vy, R* vz, int32_t* tet,
+__global__ void permute_kernel(DevParticles<R> src, R* x, R* y, R* z, double* vx, double* vy, double* vz, int32_t* tet,
                                int32_t* flags, int32_t* hit, uint32_t* pidx, const int32_t* __restrict__ perm,
                                const int32_t* __restrict__ keys_sorted, int32_t n_tets, int64_t n,
                                int64_t* __restrict__ n_active)
@@ -57,7 +57,7 @@ size_t sort_temp_bytes(int64_t n, int nbits)
 }
 
 template <typename R>
-hipError_t sort_particles(hipStream_t s, const DevParticles<R>& P, R* x, R* y, R* z, R* vx, R* vy, R* vz,
+hipError_t sort_particles(hipStream_t s, const DevParticles<R>& P, R* x, R* y, R* z, double* vx, double* vy, double* vz,
                           int32_t* tet, int32_t* flags, int32_t* hit, uint32_t* pidx, int32_t* keys,
                           int32_t* keys_out, int32_t* idx, int32_t* perm, void* temp, size_t temp_bytes,
                           int32_t n_tets, int nbits, int64_t* n_active)
@@ -77,8 +77,8 @@ template hipError_t sort_particles<double>(hipStream_t, const DevParticles<doubl
                                            double*, double*, double*, int32_t*, int32_t*, int32_t*, uint32_t*,
                                            int32_t*, int32_t*, int32_t*, int32_t*, void*, size_t, int32_t, int,
                                            int64_t*);
-template hipError_t sort_particles<float>(hipStream_t, const DevParticles<float>&, float*, float*, float*, float*,
-                                          float*, float*, int32_t*, int32_t*, int32_t*, uint32_t*, int32_t*,
+template hipError_t sort_particles<float>(hipStream_t, const DevParticles<float>&, float*, float*, float*, double*,
+                                          double*, double*, int32_t*, int32_t*, int32_t*, uint32_t*, int32_t*,
                                           int32_t*, int32_t*, int32_t*, void*, size_t, int32_t, int, int64_t*);
 
 }  // namespace sg

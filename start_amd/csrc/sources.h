@@ -20,6 +20,5 @@ struct SourceK {
 void launch_generate(hipStream_t s, const SourceK& S, uint32_t s_index, uint64_t seed, const double* centres,
                      const double* normals, double* x, double* y, double* z, double* vx, double* vy, double* vz,
                      int32_t* flags);
-void launch_to_f32(hipStream_t s, const double* a, float* b, int64_t n);
 
 }  // namespace sg

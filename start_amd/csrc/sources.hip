@@ -147,12 +147,6 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
     flags[i] = S.flags;
 }
 
-__global__ void to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t n)
-{
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
-}
-
 inline unsigned blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -164,11 +158,6 @@ void launch_generate(hipStream_t s, const SourceK& S, uint32_t s_index, uint64_t
     if (S.count > 0)
         generate_kernel<<<blocks(S.count, 256), 256, 0, s>>>(S, s_index, seed, centres, normals, x, y, z, vx, vy,
                                                               vz, flags);
-}
-
-void launch_to_f32(hipStream_t s, const double* a, float* b, int64_t n)
-{
-    if (n > 0) to_f32_kernel<<<blocks(n, 256), 256, 0, s>>>(a, b, n);
 }
 
 }  // namespace sg

@@ -52,7 +52,7 @@ void launch_abs_positions(hipStream_t s, const DevMesh& M, const float* lx, cons
                           const int32_t* tet, int64_t n, double* x, double* y, double* z);
 size_t sort_temp_bytes(int64_t n, int nbits);
 template <typename R>
-hipError_t sort_particles(hipStream_t s, const DevParticles<R>& P, R* x, R* y, R* z, R* vx, R* vy, R* vz,
+hipError_t sort_particles(hipStream_t s, const DevParticles<R>& P, R* x, R* y, R* z, double* vx, double* vy, double* vz,
                           int32_t* tet, int32_t* flags, int32_t* hit, uint32_t* pidx, int32_t* keys,
                           int32_t* keys_out, int32_t* idx, int32_t* perm, void* temp, size_t temp_bytes,
                           int32_t n_tets, int nbits, int64_t* n_active);
@@ -75,7 +75,8 @@ struct TimedStep {
     hipEvent_t a0, a1, b1;
 };
 
-// one set of particle arrays; positions/velocities are double (FP64) or float (mixed)
+// one set of particle arrays; positions are double (FP64) or float in the
+// owner's frame (mixed), velocities double in both
 struct PBuf {
     void *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
     int32_t *tet = nullptr, *flags = nullptr, *hit = nullptr;
@@ -136,7 +137,7 @@ struct start_ctx {
     void* scratch = nullptr;  // device address of the no-allocation staging array on this context's device
     std::vector<start_species_desc> species_desc;
     bool mixed = false;
-    size_t es = sizeof(double);  // bytes of one position/velocity component
+    size_t es = sizeof(double);  // bytes of one position component
     std::vector<SpeciesK> species;
     // device mesh
     double* d_nodes = nullptr;
@@ -255,7 +256,7 @@ struct start_ctx {
     template <typename R>
     DevParticles<R> view() const
     {
-        return DevParticles<R>{(R*)A.x, (R*)A.y, (R*)A.z, (R*)A.vx, (R*)A.vy, (R*)A.vz,
+        return DevParticles<R>{(R*)A.x, (R*)A.y, (R*)A.z, (double*)A.vx, (double*)A.vy, (double*)A.vz,
                                A.tet,   A.flags, A.hit,  A.pidx,  n};
     }
     StepLaunch launch_shape() const
@@ -299,8 +300,8 @@ hipError_t alloc_pbuf(PBuf& b, int64_t n, size_t es)
 {
     hipError_t e = hipSuccess;
     void** f[6] = {&b.x, &b.y, &b.z, &b.vx, &b.vy, &b.vz};
-    for (auto p : f)
-        if (e == hipSuccess) e = hipMalloc(p, es * (size_t)std::max<int64_t>(n, 1));
+    for (int k = 0; k < 6; ++k)
+        if (e == hipSuccess) e = hipMalloc(f[k], (k < 3 ? es : sizeof(double)) * (size_t)std::max<int64_t>(n, 1));
     if (e == hipSuccess) e = dalloc(&b.tet, n);
     if (e == hipSuccess) e = dalloc(&b.flags, n);
     if (e == hipSuccess) e = dalloc(&b.hit, n);
@@ -385,7 +386,7 @@ template <typename R>
 hipError_t sort_typed(start_ctx* c)
 {
     PBuf& B = c->B;
-    return sort_particles<R>(c->stream, c->view<R>(), (R*)B.x, (R*)B.y, (R*)B.z, (R*)B.vx, (R*)B.vy, (R*)B.vz, B.tet,
+    return sort_particles<R>(c->stream, c->view<R>(), (R*)B.x, (R*)B.y, (R*)B.z, (double*)B.vx, (double*)B.vy, (double*)B.vz, B.tet,
                              B.flags, B.hit, B.pidx, c->d_keys, c->d_keys_out, c->d_idx, c->d_perm, c->d_sort_temp,
                              c->sort_temp, (int32_t)c->hm.n_tets, c->sort_bits, c->d_nactive);
 }
@@ -755,7 +756,8 @@ hipError_t copy_pbuf(const PBuf& dst, const PBuf& src, int64_t n, size_t es, hip
     void* to[10] = {dst.x, dst.y, dst.z, dst.vx, dst.vy, dst.vz, dst.tet, dst.flags, dst.hit, dst.pidx};
     hipError_t e = hipSuccess;
     for (int k = 0; k < 10 && e == hipSuccess; ++k)
-        e = hipMemcpyAsync(to[k], from[k], (k < 6 ? es : 4) * (size_t)n, hipMemcpyDeviceToDevice, s);
+        e = hipMemcpyAsync(to[k], from[k], (k < 3 ? es : k < 6 ? sizeof(double) : 4) * (size_t)n,
+                           hipMemcpyDeviceToDevice, s);
     return e;
 }
 
@@ -945,7 +947,7 @@ int grow_particles(start_ctx* c, int64_t need)
 }
 
 // Rows [first, first + m) of the live buffers from host arrays: the state
-// (positions in the frame of their owner in mixed mode, velocities in FP32),
+// (positions in the frame of their owner in mixed mode, velocities FP64),
 // the flags given, hit = -1, load indices first.., and owners by the locate
 // kernel (min-id closed containment).
 int upload_rows(start_ctx* c, int64_t first, int64_t m, const double* x, const double* y, const double* z,
@@ -967,7 +969,7 @@ int upload_rows(start_ctx* c, int64_t first, int64_t m, const double* x, const d
         launch_locate(c->stream, M, (double*)c->A.x + first, (double*)c->A.y + first, (double*)c->A.z + first, m,
                       c->A.tet + first, c->A.flags + first, nullptr, nullptr, nullptr, c->d_cnt);
     } else {
-        // positions go to the frame of their owner; velocities to FP32
+        // positions go to the frame of their owner
         double *dx = nullptr, *dy = nullptr, *dz = nullptr;
         hipError_t e = dalloc(&dx, m);
         if (e == hipSuccess) e = dalloc(&dy, m);
@@ -984,13 +986,10 @@ int upload_rows(start_ctx* c, int64_t first, int64_t m, const double* x, const d
         if (dy) (void)hipFree(dy);
         if (dz) (void)hipFree(dz);
         if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("particle upload: ") + hipGetErrorString(e));
-        std::vector<float> f((size_t)m);
         const double* vs[3] = {vx, vy, vz};
         void* vd[3] = {c->A.vx, c->A.vy, c->A.vz};
-        for (int k = 0; k < 3; ++k) {
-            for (int64_t i = 0; i < m; ++i) f[(size_t)i] = (float)vs[k][i];
-            HIPCHECK(c, hipMemcpy((float*)vd[k] + first, f.data(), sizeof(float) * m, hipMemcpyHostToDevice));
-        }
+        for (int k = 0; k < 3; ++k)
+            HIPCHECK(c, hipMemcpy((double*)vd[k] + first, vs[k], b8, hipMemcpyHostToDevice));
     }
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -1406,12 +1405,12 @@ int start_generate(start_ctx* c, int32_t n_sources, const start_source_desc* src
     if (e == hipSuccess) e = dalloc(&d_nor, nor.size());
     if (e == hipSuccess) e = hipMemcpy(d_cen, cen.data(), sizeof(double) * cen.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d_nor, nor.data(), sizeof(double) * nor.size(), hipMemcpyHostToDevice);
-    // FP64 state: drawn in place; mixed: drawn into FP64 scratch, then positions
-    // to the owner's frame (by the locate) and velocities to FP32
+    // FP64 state: drawn in place; mixed: positions drawn into FP64 scratch,
+    // then to the owner's frame (by the locate); velocities in place
     double* dst[6] = {(double*)c->A.x, (double*)c->A.y, (double*)c->A.z,
                       (double*)c->A.vx, (double*)c->A.vy, (double*)c->A.vz};
     if (c->mixed)
-        for (int q = 0; q < 6 && e == hipSuccess; ++q) {
+        for (int q = 0; q < 3 && e == hipSuccess; ++q) {
             e = dalloc(&t[q], n);
             dst[q] = t[q];
         }
@@ -1426,9 +1425,6 @@ int start_generate(start_ctx* c, int32_t n_sources, const start_source_desc* src
         } else {
             launch_locate(c->stream, M, dst[0], dst[1], dst[2], n, c->A.tet, c->A.flags, (float*)c->A.x,
                           (float*)c->A.y, (float*)c->A.z, c->d_cnt);
-            launch_to_f32(c->stream, dst[3], (float*)c->A.vx, n);
-            launch_to_f32(c->stream, dst[4], (float*)c->A.vy, n);
-            launch_to_f32(c->stream, dst[5], (float*)c->A.vz, n);
         }
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1971,7 +1967,6 @@ int start_read_particles(start_ctx* c, double* x, double* y, double* z, double* 
     std::vector<uint32_t> pidx((size_t)n);
     HIPCHECK(c, hipMemcpyAsync(pidx.data(), c->A.pidx, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
     std::vector<double> tmp((size_t)n);
-    std::vector<float> ftmp((size_t)n);
     std::vector<int32_t> itmp((size_t)n);
     double* pos_src[3] = {nullptr, nullptr, nullptr};
     double* abs_buf = nullptr;
@@ -1992,18 +1987,11 @@ int start_read_particles(start_ctx* c, double* x, double* y, double* z, double* 
     hipError_t e = hipSuccess;
     for (int k = 0; k < 6 && e == hipSuccess; ++k) {
         if (!ddst[k]) continue;
-        if (k < 3 || !c->mixed) {
-            const void* src = k < 3 ? (const void*)pos_src[k] : vsrc[k - 3];
-            e = hipMemcpyAsync(tmp.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e == hipSuccess)
-                for (int64_t j = 0; j < n; ++j) ddst[k][pidx[(size_t)j]] = tmp[(size_t)j];
-        } else {
-            e = hipMemcpyAsync(ftmp.data(), vsrc[k - 3], sizeof(float) * n, hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e == hipSuccess)
-                for (int64_t j = 0; j < n; ++j) ddst[k][pidx[(size_t)j]] = (double)ftmp[(size_t)j];
-        }
+        const void* src = k < 3 ? (const void*)pos_src[k] : vsrc[k - 3];
+        e = hipMemcpyAsync(tmp.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess)
+            for (int64_t j = 0; j < n; ++j) ddst[k][pidx[(size_t)j]] = tmp[(size_t)j];
     }
     if (abs_buf) (void)hipFree(abs_buf);
     if (e != hipSuccess) return fail(c, START_ERR_HIP, std::string("read particles: ") + hipGetErrorString(e));

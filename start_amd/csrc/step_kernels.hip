@@ -704,17 +704,14 @@ __device__ inline bool collides(const StepParams& S, const SpeciesK& k, double c
 // post-collision velocity (rare: out of line to keep the streaming kernel
 // lean; arguments and result by value so nothing of the caller escapes to
 // memory)
-template <typename R>
 struct V3 {
-    R x, y, z;
+    double x, y, z;
 };
-template <typename R>
-__device__ __noinline__ V3<R> collide_velocity(int model, double mass_cp, double mass_ct, double inv_alpha, double u2,
-                                               double u3, R vx, R vy, R vz)
+__device__ __noinline__ V3 collide_velocity(int model, double mass_cp, double mass_ct, double inv_alpha, double u2,
+                                            double u3, double vx, double vy, double vz)
 {
-    CollideK<R> ck{(R)0, (R)0, (R)0, (R)0, (R)0, (R)mass_cp, (R)mass_ct, (R)inv_alpha};
-    scatter_velocity<R>(model, ck, (R)u2, (R)u3, vx, vy, vz);
-    return V3<R>{vx, vy, vz};
+    scatter_velocity(model, mass_cp, mass_ct, inv_alpha, u2, u3, vx, vy, vz);
+    return V3{vx, vy, vz};
 }
 
 // Barycentric classification with a certified threshold: strictly inside,
@@ -900,9 +897,14 @@ __device__ __forceinline__ void track_hit(const DevMesh& M, const StepParams& S,
                 xa[2] + s * D[2]);
 }
 
+// position in R (FP32 in the owner's frame in mixed mode), velocity FP64:
+// accumulated in FP32, a per-tet increment much smaller than |v| rounds the
+// same way step after step, and the velocity drifts linearly (about 6e-8 of
+// |v| per step)
 template <typename R>
 struct PState {
-    R x, y, z, vx, vy, vz;
+    R x, y, z;
+    double vx, vy, vz;
     int32_t t, f;
 };
 
@@ -925,7 +927,8 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
     const double tau = tau_of<R>(M);
     // 2. |v| < 10 m/s and 3. outside the expanded bbox: frozen this step and
     //    every later one (nothing changes its state); it still carries charge
-    R vmod = m_sqrt<R>(s.vx * s.vx + s.vy * s.vy + s.vz * s.vz);
+    const R vxr = (R)s.vx, vyr = (R)s.vy, vzr = (R)s.vz;
+    R vmod = m_sqrt<R>(vxr * vxr + vyr * vyr + vzr * vzr);
     bool frozen = vmod < R(10) || !in_bbox_at(M, S, s.t, s.x, s.y, s.z);
     double lam[4];
     if (frozen) {
@@ -944,16 +947,16 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
             // skipped, which leaves v unchanged bit for bit)
             R hx, hy, hz;
             load_push(M, k.push_slot, s.t, hx, hy, hz);
-            s.vx = (s.vx + hx) + hx;
-            s.vy = (s.vy + hy) + hy;
-            s.vz = (s.vz + hz) + hz;
+            s.vx = (s.vx + (double)hx) + (double)hx;
+            s.vy = (s.vy + (double)hy) + (double)hy;
+            s.vz = (s.vz + (double)hz) + (double)hz;
         } else if (!kBatch && k.push_slot == -2) {
             // charge 0: with m > 0 and dt > 0 the increment is the signed zero E*q
             R ex, ey, ez;
             load_E(M, s.t, ex, ey, ez);
             if (dt != R(0) && !(ex == R(0) && ey == R(0) && ez == R(0))) {
                 const R q = (R)k.charge;
-                const R hx = ex * q, hy = ey * q, hz = ez * q;
+                const double hx = (double)(ex * q), hy = (double)(ey * q), hz = (double)(ez * q);
                 s.vx = (s.vx + hx) + hx;
                 s.vy = (s.vy + hy) + hy;
                 s.vz = (s.vz + hz) + hz;
@@ -964,8 +967,20 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
             push<R>(ex, ey, ez, (R)k.charge, (R)k.mass, dt, s.vx, s.vy, s.vz);
         }
     }
-    // 6. move (Particle.cpp:157-165)
-    R x1 = s.x + s.vx * dt, y1 = s.y + s.vy * dt, z1 = s.z + s.vz * dt;
+    // 6. move (Particle.cpp:157-165).  Mixed: the end point in the owner's
+    //    frame is rounded to FP32 stochastically, with 30 hashed bits of
+    //    (particle, step, seed).
+    R x1, y1, z1;
+    if constexpr (std::is_same<R, double>::value) {
+        x1 = s.x + s.vx * S.dt;
+        y1 = s.y + s.vy * S.dt;
+        z1 = s.z + s.vz * S.dt;
+    } else {
+        const uint32_t h = hash3(vid, step, (uint32_t)S.seed ^ rk->key);
+        x1 = round_stochastic((double)s.x + s.vx * S.dt, h & 0x3ffu);
+        y1 = round_stochastic((double)s.y + s.vy * S.dt, (h >> 10) & 0x3ffu);
+        z1 = round_stochastic((double)s.z + s.vz * S.dt, h >> 22);
+    }
     if (x1 == s.x && y1 == s.y && z1 == s.z) {  // degenerate segment (:188-190)
         if (S.charge && s.t >= 0 && k.charge != 0.0) {
             bary_at(M, s.t, s.x, s.y, s.z, lam);
@@ -978,10 +993,15 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
         double u1, u1b;
         const uint32_t key = rk->key;
         variates(S.seed, key, vid, step, 0u, u1, u1b);
-        if (collides<R>(S, k, rk->conc, s.vx, s.vy, s.vz, u1)) {
+        if (collides<R>(S, k, rk->conc, (R)s.vx, (R)s.vy, (R)s.vz, u1)) {
             double u2, u3;
             variates(S.seed, key, vid, step, 1u, u2, u3);
-            V3<R> nv = collide_velocity<R>(S.model, k.mass_cp, k.mass_ct, k.inv_alpha, u2, u3, s.vx, s.vy, s.vz);
+            // Post-collision velocity with the FP64 expressions in both
+            // precisions (rare: about once per 1e4 particle-steps).  Evaluated
+            // in FP32, sin(chi) = sqrt((1 - cos)(1 + cos)) of a forward scatter
+            // (cos near 1 from an FP64 variate rounded to FP32) loses most of
+            // its digits.
+            V3 nv = collide_velocity(S.model, k.mass_cp, k.mass_ct, k.inv_alpha, u2, u3, s.vx, s.vy, s.vz);
             s.vx = nv.x;
             s.vy = nv.y;
             s.vz = nv.z;
@@ -1736,7 +1756,7 @@ __global__ void track_eligible_kernel(DevMesh M, DevParticles<R> P, StepParams S
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     const int32_t f = P.flags[i];
-    const R vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+    const R vx = (R)P.vx[i], vy = (R)P.vy[i], vz = (R)P.vz[i];
     const R vmod = m_sqrt<R>(vx * vx + vy * vy + vz * vz);
     const bool live = !(f & kSettled) && !(vmod < R(10)) && in_bbox_at(M, S, P.tet[i], P.x[i], P.y[i], P.z[i]);
     out[P.pidx[i]] = live ? 1 : 0;

@@ -312,6 +312,14 @@ class ParticleInCellStep:
         f.update(i)
         return f
 
+    def owners(self) -> tuple[np.ndarray, np.ndarray]:
+        """Owning tet and status by load index: the integer part of particles()."""
+        n = self.n
+        tet = np.empty(n, dtype=np.int32)
+        status = np.empty(n, dtype=np.int32)
+        self._check(_lib.lib().start_read_particles(self.ctx, *([None] * 6), _ip(tet), _ip(status), None))
+        return tet, status
+
     def tri_counts(self) -> np.ndarray:
         out = np.zeros(max(self.mesh.n_tris, 1), dtype=np.uint64)
         self._check(_lib.lib().start_read_tri_counts(self.ctx, _up(out)))

@@ -17,6 +17,20 @@ from start_amd.species import E_CHARGE
 REL_XV = 1e-12
 REL_Q = 1e-12
 
+# Tolerances (mixed path: FP32 positions and tables, FP64 velocities) against
+# the FP64 oracle:
+#   owner, status, hit: equal for all but 1e-3 of the particles (an end point
+#     within the FP32 position's rounding of a face can fall on the other side)
+#   x: |a - b| <= 1e-5 * |x| per particle whose owner and status agree
+#   v: |a - b| <= 1e-5 * max(|v|, rms speed) per such particle -- a collision
+#      can leave a particle far slower than the speeds its update was
+#      computed from, and the FP32 rounding of its position before then
+#      carries over to it
+#   node charge: L1 within 1e-5 of the total
+MIXED_DIFFER = 1e-3
+MIXED_REL = 1e-5
+MIXED_Q_L1 = 1e-5
+
 
 def log(msg):
     print(msg, flush=True)
@@ -123,6 +137,39 @@ class Pair:
         assert err2.max() <= REL_Q, f"{where}: tet-average density rel error {err2.max():.3e}"
         log(f"{where}: density ok (bary {err.max():.2e}, tet-average {err2.max():.2e}, "
             f"{int(wr.sum())} nodes written)")
+
+    def compare_mixed(self, where="", hit=True):
+        """The mixed-path bars above against the oracle's state; returns the
+        measured errors."""
+        g = self.eng.particles()
+        st = self.st
+        same = (g["tet"] == st.tet) & (g["status"] == st.status)
+        if hit:
+            same &= g["hit"] == st.hit
+        differ = 1.0 - same.mean()
+        xo = np.stack([st.x, st.y, st.z], 1)[same]
+        xg = np.stack([g["x"], g["y"], g["z"]], 1)[same]
+        vo = np.stack([st.vx, st.vy, st.vz], 1)
+        v_rms = float(np.sqrt(np.mean(np.sum(vo ** 2, axis=1))))
+        vo = vo[same]
+        vg = np.stack([g["vx"], g["vy"], g["vz"]], 1)[same]
+        ex = np.abs(xo - xg).max(1) / np.maximum(np.linalg.norm(xo, axis=1), 1e-300)
+        dv = np.abs(vo - vg).max(1)
+        vn = np.linalg.norm(vo, axis=1)
+        ev = dv / np.maximum(vn, v_rms)
+        res = {"differ": float(differ), "max_rel_x": float(ex.max(initial=0.0)),
+               "max_rel_v": float((dv / np.maximum(vn, 1e-300)).max(initial=0.0)),
+               "max_rel_v_rms_floor": float(ev.max(initial=0.0)), "settled": int((st.status == 1).sum())}
+        if self.charge:
+            qg = self.eng.node_charge()
+            res["charge_l1"] = float(np.abs(qg - self.qo).sum() / max(np.abs(self.qo).sum(), E_CHARGE))
+        log(f"{where}: mixed {res}")
+        assert differ <= MIXED_DIFFER, f"{where}: owner/status/hit differ for {differ:.2e} of particles"
+        assert res["max_rel_x"] <= MIXED_REL, f"{where}: max rel x {res['max_rel_x']:.2e}"
+        assert res["max_rel_v_rms_floor"] <= MIXED_REL, f"{where}: max rel v {res['max_rel_v_rms_floor']:.2e}"
+        if self.charge:
+            assert res["charge_l1"] <= MIXED_Q_L1, f"{where}: node charge L1 {res['charge_l1']:.2e}"
+        return res
 
     def close(self):
         self.eng.close()

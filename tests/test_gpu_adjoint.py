@@ -94,9 +94,10 @@ def test_torch_autograd_function():
 
 
 def test_fp32_forward_with_fp64_adjoint():
-    """Mixed forward (FP32 state and push, FP64 containment near faces), FP64
-    reverse sweep.  Bar: the gradient within 1e-4 relative of the FP64
-    reference and the moments within 1e-6 relative."""
+    """Mixed forward (FP32 positions and push increments, FP64 velocities and
+    containment near faces), FP64 reverse sweep.  Bar: the gradient within
+    1e-4 relative of the FP64 reference and the moments within 1e-6
+    relative."""
     w = _reduced()
     model = PlateDeposition(w, precision="mixed")
     xh, tri, step = model.forward(V0)

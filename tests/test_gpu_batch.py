@@ -146,9 +146,10 @@ def test_autotune_restores_state_and_keeps_parity():
 
 
 def test_mixed_batch_against_fp64_oracle():
-    """The sweep's precision (FP32 push/collision/state, FP64 containment and
-    charge) per run against the FP64 oracle: owner/status agreement > 99.5%,
-    x within 1e-5, node charge L1 within 1e-4 of each run's total."""
+    """The sweep's precision (FP32 positions and tables, FP64 velocities,
+    containment near faces and charge) per run against the FP64 oracle: owner/status for all but 1e-3 of
+    the particles, x within 1e-5, node charge L1 within 1e-5 of each run's
+    total."""
     bp = BatchPair(SIZES, SCALES, CONCS, KEYS, K=10, sort_interval=20, precision="mixed", persistent=True,
                    graph=True)
     bp.advance(100)
@@ -164,6 +165,6 @@ def test_mixed_batch_against_fp64_oracle():
         ex = (np.abs(xo - xg).max(1) / np.linalg.norm(xo, axis=1)).max()
         eq = np.abs(qg[r] - bp.qo[r]).sum() / max(np.abs(bp.qo[r]).sum(), E_CHARGE)
         log(f"mixed batch run {r}: differ {1 - same.mean():.2e} x {ex:.2e} charge L1 {eq:.2e}")
-        assert 1 - same.mean() < 5e-3 and ex < 1e-5 and eq < 1e-4
+        assert 1 - same.mean() <= 1e-3 and ex <= 1e-5 and eq <= 1e-5
         off += n
     bp.close()

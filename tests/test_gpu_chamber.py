@@ -81,20 +81,13 @@ def test_chamber_fp64_parity_with_refresh(K):
 
 
 def test_chamber_mixed_against_fp64_oracle():
-    """Mixed precision on the chamber with refreshes: owner/status/hit agree
-    for all but 0.5% of particles; node charge L1 within 1e-4 of the total."""
+    """Mixed precision on the chamber with refreshes, to the mixed bars
+    (owner/status/hit for all but 1e-3 of the particles, x and v within 1e-5,
+    node charge L1 within 1e-5)."""
     w = small_chamber()
     pr = Pair(w, E=np.zeros((w.mesh.n_tets, 3)), steps_per_launch=10, precision="mixed")
     for s in range(0, 60, w.refresh):
         refresh(pr, w, s)
         pr.advance(w.refresh)
-    g = pr.eng.particles()
-    st = pr.st
-    same = (g["tet"] == st.tet) & (g["status"] == st.status) & (g["hit"] == st.hit)
-    qg = pr.eng.node_charge()
-    eq = np.abs(qg - pr.qo).sum() / max(np.abs(pr.qo).sum(), 1e-300)
-    log(f"chamber mixed: differ {1.0 - same.mean():.2e}, charge L1 {eq:.2e}, settled gpu "
-        f"{int((g['status'] == 1).sum())} oracle {int((st.status == 1).sum())}")
-    assert 1.0 - same.mean() < 5e-3
-    assert eq < 1e-4
+    pr.compare_mixed("chamber")
     pr.close()

@@ -64,14 +64,10 @@ def test_fp64_parity_on_gmsh_meshes(name):
 
 @pytest.mark.parametrize("name", ["sphere", "plates"])
 def test_mixed_on_gmsh_meshes(name):
-    """Mixed precision on the unstructured meshes: owner/status/hit agree for
-    all but 0.5% of particles against the FP64 oracle."""
+    """Mixed precision on the unstructured meshes against the FP64 oracle, to
+    the mixed bars."""
     w = gmsh_workload(name)
     pr = Pair(w, steps_per_launch=10, precision="mixed")
     pr.advance(40)
-    g = pr.eng.particles()
-    st = pr.st
-    same = (g["tet"] == st.tet) & (g["status"] == st.status) & (g["hit"] == st.hit)
-    log(f"gmsh {name} mixed: differ {1.0 - same.mean():.2e}")
-    assert 1.0 - same.mean() < 5e-3
+    pr.compare_mixed(f"gmsh {name}")
     pr.close()

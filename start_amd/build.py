@@ -68,12 +68,17 @@ HOST = os.path.join(HERE, "host")
 HOST_LIB = os.path.join(HERE, "libstart_host.so")
 IO_LIB = os.path.join(HERE, "libstart_io.so")
 SHIM_CHECK = os.path.join(HERE, "shim_check")
+H5_LIB = os.path.join(HERE, "libstart_h5.so")
+# the image's HDF5 C library (1.10): headers and the shared library
+HDF5_PREFIX = "/opt/conda"
 
 
 def build_host(verbose: bool = False) -> list[str]:
     """The C++ step-loop entry points (libstart_host.so) over libstart_gpu.so,
     the shim_check driver that runs them like the reference's main loop, and
-    the Gmsh 4.1 reader and the output writers (libstart_io.so, host only)."""
+    the Gmsh 4.1 reader and the output writers (libstart_io.so, host only), and
+    the HDF5 deposition file (libstart_h5.so, host only, when the HDF5 C
+    library is present)."""
     gxx = shutil.which("g++") or "g++"
     common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-I", HOST]
     cmds = [
@@ -86,13 +91,24 @@ def build_host(verbose: bool = False) -> list[str]:
                           "-Wl,--no-undefined"],
     ]
     outs = [HOST_LIB, SHIM_CHECK, IO_LIB]
+    h5_inc = os.path.join(HDF5_PREFIX, "include")
+    h5_lib = os.path.join(HDF5_PREFIX, "lib")
+    if os.path.exists(os.path.join(h5_inc, "hdf5.h")) and os.path.exists(os.path.join(h5_lib, "libhdf5.so")):
+        gcc = shutil.which("gcc") or "gcc"
+        cmds.append([gcc, "-std=c11", "-O2", "-Wall", "-Wextra", "-D_POSIX_C_SOURCE=200809L", "-I",
+                     os.path.join(ROOT, "include"), "-I", h5_inc, "-fPIC", "-shared", "-o", H5_LIB + ".tmp",
+                     os.path.join(HOST, "TriangleHdf5.c"), "-L", h5_lib, "-lhdf5", "-lm", "-Wl,-rpath," + h5_lib,
+                     "-Wl,--no-undefined"])
+        outs.append(H5_LIB)
+    elif verbose:
+        print(f"no HDF5 under {HDF5_PREFIX}: libstart_h5.so not built", flush=True)
     for cmd, out in zip(cmds, outs):
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"g++ failed building {os.path.basename(out)}")
+            raise RuntimeError(f"failed building {os.path.basename(out)}")
         os.replace(out + ".tmp", out)
     return outs
 

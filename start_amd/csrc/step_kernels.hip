@@ -1248,7 +1248,9 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
             s.vz = P.vz[i];
             s.t = P.tet[i];
             const int32_t t_in = s.t, f_in = s.f;
-            const uint32_t vid = S.model != 0 ? P.pidx[i] - rk->first : 0u;
+            // variate counter: collisions need it; the mixed path's stochastic
+            // rounding keys on it too
+            const uint32_t vid = (S.model != 0 || !std::is_same<R, double>::value) ? P.pidx[i] - rk->first : 0u;
             bool moved = false;
             for (int ks = 0; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
@@ -1479,7 +1481,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
         uint32_t settle_step = step0 + (uint32_t)it.k;  // step and start of the settling segment
         double sx = x0[0], sy = x0[1], sz = x0[2];
         if (!settled) {
-            const uint32_t vid = S.model != 0 ? P.pidx[i] - rk->first : 0u;
+            const uint32_t vid = (S.model != 0 || !std::is_same<R, double>::value) ? P.pidx[i] - rk->first : 0u;
             unsigned long long extra = 0;
             for (int ks = it.k + 1; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;

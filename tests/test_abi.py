@@ -19,7 +19,8 @@ def _declared(header="start_gpu.h"):
 
 
 def test_every_header_is_checked():
-    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["start_gpu.h", "start_mesh.h", "start_output.h"]
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == ["start_gpu.h", "start_hdf5.h", "start_mesh.h",
+                                                                  "start_output.h"]
 
 
 def test_io_library_exports_every_declared_symbol():
@@ -27,6 +28,19 @@ def test_io_library_exports_every_declared_symbol():
     lib = C.CDLL(os.path.join(ROOT, "start_amd", "libstart_io.so"))
     names = _declared("start_mesh.h") + _declared("start_output.h")
     assert "start_gmsh_read" in names and "start_write_movements_json" in names and len(names) == 11
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_hdf5_library_exports_every_declared_symbol():
+    """libstart_h5.so (host only, over the image's HDF5 C library)."""
+    path = os.path.join(ROOT, "start_amd", "libstart_h5.so")
+    if not os.path.exists(path):
+        pytest.skip("libstart_h5.so not built: no HDF5 C library at build time")
+    lib = C.CDLL(path)
+    names = _declared("start_hdf5.h")
+    assert names == ["start_hdf5_last_error", "start_read_triangle_hdf5", "start_triangle_area",
+                     "start_write_settled_hdf5", "start_write_triangle_hdf5"]
     for name in names:
         assert hasattr(lib, name), name
 

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libstart_gpu.so")
 
 START_OK = 0
+START_ERR_NOMEM = -6  # device allocation failed (include/start_gpu.h)
 MODEL = {"NONE": 0, "HS": 1, "VHS": 2, "VSS": 3}
 PREC_FP64, PREC_MIXED = 0, 1
 ACTIVE, SETTLED = 0, 1

@@ -288,12 +288,36 @@ int fail(start_ctx* c, int code, const std::string& msg)
             return fail((ctx), START_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device allocation that leaves no error behind when it fails: HIP keeps a
+// failed hipMalloc's error as the thread's last error, which the next
+// hipGetLastError() after a kernel launch would report as the launch's.
+hipError_t dev_malloc(void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t host_pinned_malloc(void** p, size_t bytes)
+{
+    hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+    }
+    return e;
+}
+
 template <typename T>
 hipError_t dalloc(T** p, size_t count)
 {
     *p = nullptr;
     if (count == 0) count = 1;
-    return hipMalloc((void**)p, sizeof(T) * count);
+    if (count > SIZE_MAX / sizeof(T)) return hipErrorOutOfMemory;
+    return dev_malloc((void**)p, sizeof(T) * count);
 }
 
 hipError_t alloc_pbuf(PBuf& b, int64_t n, size_t es)
@@ -301,7 +325,7 @@ hipError_t alloc_pbuf(PBuf& b, int64_t n, size_t es)
     hipError_t e = hipSuccess;
     void** f[6] = {&b.x, &b.y, &b.z, &b.vx, &b.vy, &b.vz};
     for (int k = 0; k < 6; ++k)
-        if (e == hipSuccess) e = hipMalloc(f[k], (k < 3 ? es : sizeof(double)) * (size_t)std::max<int64_t>(n, 1));
+        if (e == hipSuccess) e = dev_malloc(f[k], (k < 3 ? es : sizeof(double)) * (size_t)std::max<int64_t>(n, 1));
     if (e == hipSuccess) e = dalloc(&b.tet, n);
     if (e == hipSuccess) e = dalloc(&b.flags, n);
     if (e == hipSuccess) e = dalloc(&b.hit, n);
@@ -362,7 +386,7 @@ hipError_t alloc_work_buffers(start_ctx* c, int64_t cap)
         while (bits < 31 && ((int64_t)1 << bits) <= c->hm.n_tets + 1) ++bits;
         c->sort_bits = bits;
         c->sort_temp = sort_temp_bytes(cap, bits);
-        he = hipMalloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
+        he = dev_malloc(&c->d_sort_temp, std::max<size_t>(c->sort_temp, 16));
     }
     return he;
 }
@@ -1040,12 +1064,12 @@ int assign_track_slots(start_ctx* c, int64_t first)
         }
     const int64_t m_new = (int64_t)c->trk_rows.size();
     if (m_new == m_old) return START_OK;
+    // Every buffer first, the slots last: a step launched after a failure
+    // here must not find a slot without its record space.
     if (!c->d_trk_slot) {
         e = dalloc(&c->d_trk_slot, std::max(c->cap, n));
         if (e == hipSuccess) e = hipMemset(c->d_trk_slot, 0xff, sizeof(int32_t) * (size_t)std::max(c->cap, n));
     }
-    if (e == hipSuccess)
-        e = hipMemcpy(c->d_trk_slot + first, slot.data(), sizeof(int32_t) * (size_t)(n - first), hipMemcpyHostToDevice);
     int32_t* cnt2 = nullptr;
     double* xyz2 = nullptr;
     const size_t per = 3 * (size_t)c->trk_max;
@@ -1057,10 +1081,14 @@ int assign_track_slots(start_ctx* c, int64_t first)
         if (e == hipSuccess)
             e = hipMemcpy(xyz2, c->d_trk_xyz, sizeof(double) * (size_t)m_old * per, hipMemcpyDeviceToDevice);
     }
+    if (e == hipSuccess)
+        e = hipMemcpy(c->d_trk_slot + first, slot.data(), sizeof(int32_t) * (size_t)(n - first), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         if (cnt2) (void)hipFree(cnt2);
         if (xyz2) (void)hipFree(xyz2);
         c->trk_rows.resize((size_t)m_old);
+        if (m_old == 0) free_tracking(c);  // nothing was tracked: tracking stays off
+        else (void)hipMemset(c->d_trk_slot + first, 0xff, sizeof(int32_t) * (size_t)(n - first));
         return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
                     std::string("movement tracking: ") + hipGetErrorString(e));
     }
@@ -1589,7 +1617,7 @@ int start_set_node_potential_basis(start_ctx* c, int32_t which, const double* ph
     const size_t bytes = sizeof(double) * (size_t)c->hm.n_nodes;
     if (!c->d_phi) {
         HIPCHECK(c, dalloc(&c->d_phi, c->hm.n_nodes));
-        HIPCHECK(c, hipHostMalloc((void**)&c->h_phi, std::max<size_t>(bytes, 8), hipHostMallocDefault));
+        HIPCHECK(c, host_pinned_malloc((void**)&c->h_phi, std::max<size_t>(bytes, 8)));
         HIPCHECK(c, hipEventCreateWithFlags(&c->phi_copied, hipEventDisableTiming));
     }
     if (c->phi_pending) {
@@ -1926,9 +1954,9 @@ int start_node_charge_density(start_ctx* c, int32_t mode, double* rho, int32_t* 
     if (!c->d_rho) {
         hipError_t e = dalloc(&c->d_rho, M);
         if (e == hipSuccess) e = dalloc(&c->d_written, M);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_rho, sizeof(double) * (size_t)M, hipHostMallocDefault);
+        if (e == hipSuccess) e = host_pinned_malloc((void**)&c->h_rho, sizeof(double) * (size_t)M);
         if (e == hipSuccess)
-            e = hipHostMalloc((void**)&c->h_written, sizeof(int32_t) * (size_t)M, hipHostMallocDefault);
+            e = host_pinned_malloc((void**)&c->h_written, sizeof(int32_t) * (size_t)M);
         if (e != hipSuccess)
             return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
                         std::string("node charge density buffers: ") + hipGetErrorString(e));

@@ -1,14 +1,14 @@
 """Edge cases on the GPU against the oracle: an empty population, a single
 particle, a population frozen from the start (it still carries charge), a
-batch with an empty run, and movement records across a batch and through
-the autotuner."""
+batch with an empty run, movement records across a batch and through the
+autotuner, and a device allocation that fails."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
 from oracle import pyoracle as O
-from start_amd import workloads
+from start_amd import _lib, workloads
 from start_amd.engine import ParticleInCellStep
 
 from ._parity import REL_XV, Pair, log
@@ -54,6 +54,29 @@ def test_frozen_population_still_deposits_charge():
     r = pr.compare("frozen population")
     assert r["processed"] == 0 and r["max_rel_q"] <= 1e-12
     assert np.abs(pr.qo).sum() > 0
+    pr.close()
+
+
+def test_out_of_device_memory_is_reported_and_the_context_stays_usable():
+    """Movement records of 2e9 points for each of 1,000 particles (48 TB):
+    the allocation fails with START_ERR_NOMEM, tracking stays off, and the
+    context steps on bit-exact against the oracle and tracks again when asked
+    for a size that fits."""
+    w = workloads.small_box(n_particles=1000, charge_units=2.0)
+    pr = Pair(w, steps_per_launch=1)
+    pr.advance(3)
+    log("edge: asking for 48 TB of movement records")
+    with pytest.raises(_lib.StartError) as e:
+        pr.eng.track_movements(max_particles=1000, max_points=2_000_000_000)
+    assert e.value.code == _lib.START_ERR_NOMEM, e.value
+    assert len(pr.eng.movement_counts()[0]) == 0
+    pr.advance(10)
+    pr.compare("after a failed allocation")
+    pr.eng.track_movements(max_particles=100, max_points=8)
+    pr.advance(3)
+    ids, counts = pr.eng.movement_counts()
+    assert len(ids) == 100 and (counts >= 3).all() and (counts <= 4).all()  # 3 starts, and a hit point
+    pr.compare("tracking after a failed allocation")
     pr.close()
 
 

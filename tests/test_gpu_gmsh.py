@@ -21,7 +21,12 @@ G = os.path.join(os.path.dirname(__file__), "golden", "gmsh")
 
 
 def gmsh_workload(name, n=4000, seed=17):
-    mesh = gmsh.read(os.path.join(G, name + ".msh")).mesh
+    return mesh_workload(gmsh.read(os.path.join(G, name + ".msh")).mesh, name, n, seed)
+
+
+def mesh_workload(mesh, name, n=4000, seed=17):
+    """Ar+ uniform in the mesh, steps of about a sixth of a typical edge, a few
+    % collision chance per step, a linear potential along z."""
     om = O.Mesh(mesh.nodes, mesh.tets, mesh.tris)
     rng = np.random.default_rng(seed)
     lo, hi = mesh.nodes.min(axis=0), mesh.nodes.max(axis=0)

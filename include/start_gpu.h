@@ -244,6 +244,17 @@ int start_get_launch(start_ctx* ctx, start_launch_cfg* cfg);
  * for trial_steps steps from a copy of the current state, restores the
  * state, and sets the fastest.  best and ms_per_step may be NULL. */
 int start_autotune(start_ctx* ctx, int32_t trial_steps, start_launch_cfg* best, double* ms_per_step);
+/* start_autotune with a cache file of results: a context of the same shape --
+ * device (gcnArchName and CU count), precision, collision model, em_push,
+ * charge every step, runs, tets, particle count to the next power of two, and
+ * trial_steps -- takes the cached configuration without trials
+ * (*from_cache = 1); otherwise the trials run and their result is appended to
+ * the file (*from_cache = 0).  One line per result, "key|K block persistent
+ * sort_interval walk|ms"; the last line of a key wins.  Callers that share a
+ * file append without locking.  best, ms_per_step and from_cache may be NULL.
+ * (No counterpart in the reference, which has no launch shapes.) */
+int start_autotune_cached(start_ctx* ctx, const char* cache_path, int32_t trial_steps, start_launch_cfg* best,
+                          double* ms_per_step, int32_t* from_cache);
 
 /* ---- particle movements (the particleMovementMap argument of process) ----
  * Replaces ParticleMovementTracker::recordMovement

@@ -27,7 +27,7 @@ EXPORTS = [
     "start_node_charge_density", "start_set_gas", "start_set_time_step",
     "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
-    "start_autotune", "start_record_settle", "start_read_hits", "start_adjoint_push",
+    "start_autotune", "start_autotune_cached", "start_record_settle", "start_read_hits", "start_adjoint_push",
     "start_read_tet_field", "start_generate", "start_track_movements", "start_read_movements",
     "start_clear_movements", "start_locate_noalloc", "start_append_particles", "start_mesh_maps", "start_timing_read_each", "start_read_settled_since",
 ]
@@ -139,6 +139,7 @@ def lib() -> C.CDLL:
     L.start_set_launch.argtypes = [vp, P(LaunchCfg)]
     L.start_get_launch.argtypes = [vp, P(LaunchCfg)]
     L.start_autotune.argtypes = [vp, i32, P(LaunchCfg), P(d)]
+    L.start_autotune_cached.argtypes = [vp, C.c_char_p, i32, P(LaunchCfg), P(d), P(i32)]
     L.start_record_settle.argtypes = [vp, i32]
     L.start_read_hits.argtypes = [vp, P(d), P(i32), P(i32)]
     L.start_adjoint_push.argtypes = [vp, P(d), i32, P(d)]

@@ -134,6 +134,7 @@ struct start_ctx {
     start_params prm{};
     start_launch_cfg cfg{};
     int n_cu = 256;
+    std::string arch = "unknown";  // the device's gcnArchName, part of the autotuner's cache key
     void* scratch = nullptr;  // device address of the no-allocation staging array on this context's device
     std::vector<start_species_desc> species_desc;
     bool mixed = false;
@@ -1155,6 +1156,8 @@ int start_create(start_ctx** out, const start_mesh_desc* mesh, const start_param
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
             c->n_cu = ncu;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) c->arch = prop.gcnArchName;
     }
     c->mixed = params->precision == START_PREC_MIXED;
     c->es = c->mixed ? sizeof(float) : sizeof(double);
@@ -1784,6 +1787,82 @@ int start_autotune(start_ctx* c, int32_t trial_steps, start_launch_cfg* best, do
     if (best) *best = best_cfg;
     if (ms_per_step) *ms_per_step = best_ms;
     return check_kernel_errors(c);
+}
+
+namespace {
+
+// The autotuner's cache key: what the launch shape's speed depends on -- the
+// device, the precision, the collision model and the work per step (the
+// particle count by power of two, the tets, the runs) -- and the trial length.
+std::string autotune_key(const start_ctx* c, int32_t trial_steps)
+{
+    int64_t bucket = 1;
+    while (bucket < c->n) bucket <<= 1;
+    std::string k = "arch=" + c->arch + ";cu=" + std::to_string(c->n_cu) + ";prec=" + (c->mixed ? "mixed" : "fp64") +
+                    ";model=" + std::to_string(c->prm.gas.model) + ";em=" + std::to_string(c->prm.em_push) +
+                    ";charge=" + std::to_string(c->prm.charge_every_step) + ";runs=" + std::to_string(c->n_runs()) +
+                    ";tets=" + std::to_string(c->hm.n_tets) + ";n<=" + std::to_string(bucket) +
+                    ";trial=" + std::to_string(trial_steps);
+    for (char& ch : k)
+        if (ch == ' ' || ch == '\n' || ch == '|') ch = '_';
+    return k;
+}
+
+}  // namespace
+
+int start_autotune_cached(start_ctx* c, const char* cache_path, int32_t trial_steps, start_launch_cfg* best,
+                          double* ms_per_step, int32_t* from_cache)
+{
+    if (!c) return fail(nullptr, START_ERR_ARG, "null context");
+    if (!cache_path) return fail(c, START_ERR_ARG, "null cache path");
+    if (trial_steps < 1) return fail(c, START_ERR_ARG, "trial_steps must be >= 1");
+    if (c->n == 0) return fail(c, START_ERR_STATE, "no particles loaded");
+    const std::string key = autotune_key(c, trial_steps);
+    // lines "key|K block persistent sort_interval walk|ms"; the last match wins
+    bool found = false;
+    start_launch_cfg g = c->cfg;
+    double ms = 0.0;
+    if (FILE* f = fopen(cache_path, "r")) {
+        char line[1024];
+        while (fgets(line, sizeof line, f)) {
+            const char* bar = strchr(line, '|');
+            if (!bar || std::string(line, (size_t)(bar - line)) != key) continue;
+            start_launch_cfg h = c->cfg;
+            double t = 0.0;
+            if (sscanf(bar + 1, "%d %d %d %d %d|%lf", &h.steps_per_launch, &h.block, &h.persistent,
+                       &h.sort_interval, &h.walk, &t) == 6 &&
+                h.steps_per_launch >= 1 && (h.block == 128 || h.block == 256) && (h.persistent == 0 || h.persistent == 1) &&
+                h.sort_interval >= 0 && (h.walk == 0 || h.walk == 1)) {
+                g = h;
+                ms = t;
+                found = true;
+            }
+        }
+        fclose(f);
+    }
+    if (found) {
+        g.graph = c->cfg.graph;
+        int rc = start_set_launch(c, &g);
+        if (rc != START_OK) return rc;
+        if (best) *best = g;
+        if (ms_per_step) *ms_per_step = ms;
+        if (from_cache) *from_cache = 1;
+        return START_OK;
+    }
+    start_launch_cfg b{};
+    int rc = start_autotune(c, trial_steps, &b, &ms);
+    if (rc != START_OK) return rc;
+    if (FILE* f = fopen(cache_path, "a")) {
+        fprintf(f, "%s|%d %d %d %d %d|%.6g\n", key.c_str(), b.steps_per_launch, b.block, b.persistent, b.sort_interval,
+                b.walk, ms);
+        fclose(f);
+    } else {
+        return fail(c, START_ERR_ARG, std::string("cannot append to the autotune cache ") + cache_path);
+    }
+    if (best) *best = b;
+    if (ms_per_step) *ms_per_step = ms;
+    if (from_cache) *from_cache = 0;
+    return START_OK;
 }
 
 int start_track_movements(start_ctx* c, int64_t max_particles, int32_t max_points)

@@ -13,6 +13,7 @@ ParticleDynamicsProcessor.cpp:202-209; the C ABI reports both as status codes).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -201,13 +202,21 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_get_launch(self.ctx, C.byref(cfg)))
         return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_}
 
-    def autotune(self, trial_steps: int = 200):
+    def autotune(self, trial_steps: int = 200, cache: str | None = None):
         """Project autotuner: times the launch configurations from the current
         state (restored afterwards) and keeps the fastest; returns it and its
-        ms per step."""
+        ms per step.  With a cache file, a context of the same shape on the
+        same device takes the cached result without trials (self.autotune_hit)."""
         cfg = _lib.LaunchCfg()
         ms = C.c_double(0)
-        self._check(_lib.lib().start_autotune(self.ctx, int(trial_steps), C.byref(cfg), C.byref(ms)))
+        if cache is None:
+            self._check(_lib.lib().start_autotune(self.ctx, int(trial_steps), C.byref(cfg), C.byref(ms)))
+            self.autotune_hit = False
+        else:
+            hit = C.c_int32(0)
+            self._check(_lib.lib().start_autotune_cached(self.ctx, os.fsencode(cache), int(trial_steps),
+                                                         C.byref(cfg), C.byref(ms), C.byref(hit)))
+            self.autotune_hit = bool(hit.value)
         return {k: getattr(cfg, k) for k, _ in _lib.LaunchCfg._fields_}, ms.value
 
     def record_settle(self, on: bool = True):

@@ -145,11 +145,38 @@ def test_autotune_restores_state_and_keeps_parity():
     bp.close()
 
 
+def test_autotune_cache(tmp_path):
+    """start_autotune_cached: the first context tunes and appends its result;
+    a second context of the same shape takes it without trials and stays
+    bit-exact; a different trial length is another key."""
+    cache = str(tmp_path / "autotune.txt")
+    shape = ([4000, 2500], [(500.0, 0.0), (200.0, 10.0)], [2.4e22, 6.0e21], [3, 4])
+    bp = BatchPair(*shape, K=1, sort_interval=10)
+    bp.advance(5)
+    cfg, ms = bp.eng.autotune(trial_steps=40, cache=cache)
+    assert not bp.eng.autotune_hit and ms > 0
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and "arch=gfx950" in lines[0] and ";trial=40" in lines[0], lines
+    bp.close()
+    bp = BatchPair(*shape, K=1, sort_interval=10)
+    bp.advance(5)
+    cfg2, ms2 = bp.eng.autotune(trial_steps=40, cache=cache)
+    assert bp.eng.autotune_hit and cfg2 == cfg and ms2 == pytest.approx(ms, rel=1e-5)
+    assert bp.eng.get_launch() == cfg
+    bp.K = cfg["steps_per_launch"]
+    bp.advance(50)
+    bp.compare("after a cached autotune")
+    bp.eng.autotune(trial_steps=50, cache=cache)
+    assert not bp.eng.autotune_hit and len(open(cache).read().splitlines()) == 2
+    log(f"autotune cache: {cfg} at {ms:.4f} ms/step, taken from the cache by the second context")
+    bp.close()
+
+
 def test_mixed_batch_against_fp64_oracle():
     """The sweep's precision (FP32 positions and tables, FP64 velocities,
-    containment near faces and charge) per run against the FP64 oracle: owner/status for all but 1e-3 of
-    the particles, x within 1e-5, node charge L1 within 1e-5 of each run's
-    total."""
+    containment near faces and charge) per run against the FP64 oracle:
+    owner/status for all but 1e-3 of the particles, x within 1e-5, node
+    charge L1 within 1e-5 of each run's total."""
     bp = BatchPair(SIZES, SCALES, CONCS, KEYS, K=10, sort_interval=20, precision="mixed", persistent=True,
                    graph=True)
     bp.advance(100)

@@ -7,9 +7,11 @@
  * triangle, element type 2, keyed by its element tag).  ASCII and binary
  * 4.1 files; other element types are skipped.
  *
- * Rows: nodes in ascending tag order; tets and triangles in file order, their
- * node references as node rows (0-based), their element tags alongside.  The
- * arrays are allocated with malloc and released by start_gmsh_free. */
+ * Rows: nodes, tets and triangles each in ascending tag order (the device
+ * breaks owner and hit ties by the smaller row, the reference by the smaller
+ * element tag); element node references as node rows (0-based), element tags
+ * alongside.  A repeated tag is an error.  The arrays are allocated with
+ * malloc and released by start_gmsh_free. */
 #ifndef START_MESH_H
 #define START_MESH_H
 
@@ -25,10 +27,10 @@ typedef struct start_gmsh_mesh {
     uint64_t* node_tags; /* n_nodes, ascending                 */
     int64_t n_tets;
     int32_t* tets;       /* 4 * n_tets node rows               */
-    uint64_t* tet_tags;  /* n_tets                             */
+    uint64_t* tet_tags;  /* n_tets, ascending (rows sorted by tag) */
     int64_t n_tris;
     int32_t* tris;       /* 3 * n_tris node rows               */
-    uint64_t* tri_tags;  /* n_tris: the reference's triangle ids */
+    uint64_t* tri_tags;  /* n_tris, ascending: the reference's triangle ids */
 } start_gmsh_mesh;
 
 /* 0 on success; -1 with the reason in start_gmsh_last_error() otherwise. */

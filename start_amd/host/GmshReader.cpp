@@ -286,7 +286,28 @@ int start_gmsh_read(const char* path, start_gmsh_mesh* out)
             }
             return v;
         };
-        const std::vector<int32_t> tets = rows(r.tet_nodes), tris = rows(r.tri_nodes);
+        std::vector<int32_t> tets = rows(r.tet_nodes), tris = rows(r.tri_nodes);
+        // Elements in ascending tag order: the device breaks owner ties (a point
+        // on a shared face) by the smaller row and hit ties by the smaller
+        // triangle row, where the reference takes the smaller element tag
+        // (its maps are keyed by tag: NodeChargeDensityProcessor.cpp:51-56).
+        auto by_tag = [](std::vector<uint64_t>& tag, std::vector<int32_t>& conn, int per, const char* what) {
+            std::vector<size_t> ord(tag.size());
+            for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+            std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return tag[a] < tag[b]; });
+            std::vector<uint64_t> t2(tag.size());
+            std::vector<int32_t> c2(conn.size());
+            for (size_t k = 0; k < ord.size(); ++k) {
+                t2[k] = tag[ord[k]];
+                if (k && t2[k] == t2[k - 1])
+                    throw std::runtime_error(std::string("duplicate ") + what + " tag " + std::to_string(t2[k]));
+                for (int q = 0; q < per; ++q) c2[per * k + q] = conn[per * ord[k] + q];
+            }
+            tag.swap(t2);
+            conn.swap(c2);
+        };
+        by_tag(r.tet_tag, tets, 4, "tetrahedron");
+        by_tag(r.tri_tag, tris, 3, "triangle");
         out->n_nodes = (int64_t)nn;
         out->nodes = copy_out(xyz);
         out->node_tags = copy_out(tags);

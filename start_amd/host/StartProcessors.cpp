@@ -99,8 +99,26 @@ ParticleKernelDevice::ParticleKernelDevice(DeviceMeshInput mesh, DeviceKernelOpt
         for (size_t n = 0; n < M; ++n) m_mesh.nodeTags.push_back(n + 1);  // Gmsh tags 1..M (Q19)
     if (m_mesh.triangleTags.empty())
         for (size_t s = 0; s < S; ++s) m_mesh.triangleTags.push_back(s);
-    if (m_mesh.nodeTags.size() != M || m_mesh.triangleTags.size() != S)
+    if (m_mesh.nodeTags.size() != M || m_mesh.triangleTags.size() != S ||
+        (!m_mesh.tetTags.empty() && m_mesh.tetTags.size() != m_mesh.tets.size() / 4))
         throw std::invalid_argument("tag arrays do not match the mesh");
+    // rows in ascending tag order (ties of owner and hit go to the smaller tag)
+    auto by_tag = [](std::vector<size_t>& tag, std::vector<int32_t>& conn, size_t per) {
+        std::vector<size_t> ord(tag.size());
+        for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return tag[a] < tag[b]; });
+        std::vector<size_t> t2(tag.size());
+        std::vector<int32_t> c2(conn.size());
+        for (size_t k = 0; k < ord.size(); ++k) {
+            t2[k] = tag[ord[k]];
+            if (k && t2[k] == t2[k - 1]) throw std::invalid_argument("repeated element tag " + std::to_string(t2[k]));
+            for (size_t q = 0; q < per; ++q) c2[per * k + q] = conn[per * ord[k] + q];
+        }
+        tag.swap(t2);
+        conn.swap(c2);
+    };
+    if (!m_mesh.tetTags.empty()) by_tag(m_mesh.tetTags, m_mesh.tets, 4);
+    by_tag(m_mesh.triangleTags, m_mesh.triangles, 3);
     m_rowsByTag.resize(M);
     for (size_t n = 0; n < M; ++n) m_rowsByTag[n] = n;
     std::sort(m_rowsByTag.begin(), m_rowsByTag.end(),

@@ -115,6 +115,10 @@ struct DeviceMeshInput {
     std::vector<int32_t> triangles; // 3 node rows per surface triangle
     std::vector<size_t> nodeTags;   // tag of each node row (empty: row + 1)
     std::vector<size_t> triangleTags; // tag of each triangle row (empty: row)
+    std::vector<size_t> tetTags;    // tag of each tet row (empty: rows ascend with the tags)
+    // Tets and triangles are reordered by tag on construction: the device
+    // breaks owner and hit ties by the smaller row, the reference by the
+    // smaller element tag (NodeChargeDensityProcessor.cpp:51-56).
 };
 
 class ParticleKernelDevice {

@@ -175,3 +175,27 @@ def test_errors(tmp_path):
                  "$Elements\n1 1 1 1\n2 1 2 1\n1 7 8 9\n$EndElements\n")
     with pytest.raises(RuntimeError, match="unknown node tag"):
         gmsh.read(str(p))
+
+
+def test_elements_come_back_in_tag_order(tmp_path):
+    """A file whose element tags do not ascend in file order (a renumbered or
+    merged mesh): tets and triangles come back sorted by tag, with their node
+    rows, so the device's smallest-row tie-break is the reference's
+    smallest-tag one (NodeChargeDensityProcessor.cpp:51-56)."""
+    from start_amd import meshgen
+    m = meshgen.box_kuhn(2, 2, 2, 0.2, 0.2, 0.2)
+    rng = np.random.default_rng(3)
+    tet_tags = rng.permutation(np.arange(1000, 1000 + m.n_tets)).astype(np.uint64)
+    tri_tags = rng.permutation(np.arange(1, 1 + m.n_tris)).astype(np.uint64)
+    p = str(tmp_path / "permuted.msh")
+    gmsh.write(p, m, None, tet_tags, tri_tags)
+    g = gmsh.read(p)
+    ot, os_ = np.argsort(tet_tags), np.argsort(tri_tags)
+    assert np.array_equal(g.tet_tags, tet_tags[ot]) and np.array_equal(g.mesh.tets, m.tets[ot])
+    assert np.array_equal(g.tri_tags, tri_tags[os_]) and np.array_equal(g.mesh.tris, m.tris[os_])
+    # a repeated tet tag is an error
+    dup = tet_tags.copy()
+    dup[1] = dup[0]
+    gmsh.write(p, m, None, dup, tri_tags)
+    with pytest.raises(RuntimeError, match="duplicate tetrahedron tag"):
+        gmsh.read(p)

@@ -83,7 +83,8 @@ def build_host(verbose: bool = False) -> list[str]:
     common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-I", HOST]
     cmds = [
         [gxx] + common + ["-fPIC", "-shared", "-o", HOST_LIB + ".tmp", os.path.join(HOST, "StartProcessors.cpp"),
-                          "-L", HERE, "-lstart_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"],
+                          os.path.join(HOST, "DeviceConfig.cpp"), "-L", HERE, "-lstart_gpu", "-Wl,-rpath,$ORIGIN",
+                          "-Wl,--no-undefined"],
         [gxx] + common + ["-o", SHIM_CHECK + ".tmp", os.path.join(HOST, "shim_check.cpp"),
                           "-L", HERE, "-lstart_host", "-lstart_gpu", "-Wl,-rpath,$ORIGIN"],
         [gxx] + common + ["-fPIC", "-shared", "-o", IO_LIB + ".tmp", os.path.join(HOST, "GmshReader.cpp"),

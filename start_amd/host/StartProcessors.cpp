@@ -127,6 +127,13 @@ ParticleKernelDevice::ParticleKernelDevice(DeviceMeshInput mesh, DeviceKernelOpt
     m_written.resize(M);
 }
 
+void ParticleKernelDevice::useConfig(std::string_view path)
+{
+    if (m_config_read || m_ctx || path.empty()) return;
+    m_opt = deviceOptionsFromConfig(std::string(path), m_opt);
+    m_config_read = true;
+}
+
 ParticleKernelDevice::~ParticleKernelDevice()
 {
     if (m_ctx) start_destroy(m_ctx);
@@ -371,7 +378,7 @@ std::map<size_t, uint64_t> ParticleKernelDevice::triangleCounts()
 
 // ---------------------------------------------------------------------------
 
-void ParticleDynamicsProcessor::process(std::string_view /*config_filename*/, ParticleVector& particles,
+void ParticleDynamicsProcessor::process(std::string_view config_filename, ParticleVector& particles,
                                         double /*timeMoment*/, double timeStep, unsigned int /*numThreads*/,
                                         ParticleKernelDevice& device, ParticleTrackerMap& /*particleTracker*/,
                                         ParticlesIDSet& settledParticlesIds,
@@ -382,6 +389,7 @@ void ParticleDynamicsProcessor::process(std::string_view /*config_filename*/, Pa
                                         double gasConcentration)
 {
     try {
+        device.useConfig(config_filename);
         device.ensureContext();
         device.configure(scatteringModel, gasName, gasConcentration, timeStep);
         device.ensureParticles(particles);
@@ -395,13 +403,14 @@ void ParticleDynamicsProcessor::process(std::string_view /*config_filename*/, Pa
     }
 }
 
-void NodeChargeDensityProcessor::gather(double /*timeMoment*/, std::string_view /*configFilename*/,
+void NodeChargeDensityProcessor::gather(double /*timeMoment*/, std::string_view configFilename,
                                         ParticleKernelDevice& device, ParticleVector const& particles,
                                         ParticlesIDSet& /*settledParticleIDs*/,
                                         ParticleTrackerMap& /*particleTrackerMap*/,
                                         NodeChargeDensitiesMap& nodeChargeDensityMap)
 {
     try {
+        device.useConfig(configFilename);
         device.ensureContext();
         device.ensureParticles(particles);
         device.densities(nodeChargeDensityMap);

@@ -107,6 +107,22 @@ struct DeviceKernelOptions {
     std::map<ParticleType, double> chargeOverride;
 };
 
+// Options from the run's JSON configuration file (the file ConfigParser reads,
+// ConfigParser.cpp:18-226), applied over `options`.  Optional keys the
+// reference does not know and ignores:
+//   "Device Precision": "FP64" | "Mixed"
+//   "Device EM Push": true | false           (false: the reference's literal loop, Q1)
+//   "Device Seed": integer                    (variate stream key)
+//   "Device Density Mode": "Barycentric" | "TetAverage"
+//   "Device Sort Interval": integer           (steps between sorts by tet; 0 never)
+//   "Device Write Back Each Step": true | false
+//   "Device Movement Max Particles": integer
+//   "Device Charges": {"Ti": 2, ...}          (elementary charges by particle type)
+//   "Device Diameters": {"Ti": 2.1e-10, ...}  (collision diameters r_p + r_t in m)
+// The reference's own keys are skipped; an unknown "Device ..." key, a value of
+// the wrong kind or a malformed file throws std::runtime_error.
+DeviceKernelOptions deviceOptionsFromConfig(const std::string& path, DeviceKernelOptions options = {});
+
 // The mesh as the step needs it: the tetrahedra of VolumetricMesh and the
 // surface triangles of SurfaceMesh, with their Gmsh tags (GmshUtils.cpp:116-136).
 struct DeviceMeshInput {
@@ -147,6 +163,9 @@ private:
     // The context is created at the first process() / gather() call with one
     // species per ParticleType; the gas and time step follow each process().
     void ensureContext();
+    // Reads the "Device ..." keys of the configuration file once, before the
+    // context exists (process() and gather() pass their config_filename).
+    void useConfig(std::string_view path);
     void configure(std::string_view scatteringModel, std::string_view gasName, double gasConcentration,
                    double timeStep);
     // Loads the ParticleVector at the first call; afterwards appends the
@@ -168,6 +187,7 @@ private:
     std::vector<int32_t> m_written;
     start_ctx* m_ctx = nullptr;
     bool m_configured = false;
+    bool m_config_read = false;
     double m_dt = 0.0;
     std::string m_model, m_gas;
     double m_conc = 0.0;

@@ -60,10 +60,34 @@ static void write_array(const std::string& path, const std::vector<T>& v)
     if (!f) throw std::runtime_error("cannot write " + path);
 }
 
+// --print-config <file.json>: the device options deviceOptionsFromConfig reads
+// from a run configuration, one "name value" line each (no GPU needed)
+int print_config(const char* path)
+{
+    try {
+        const DeviceKernelOptions o = deviceOptionsFromConfig(path);
+        std::cout.precision(17);
+        std::cout << "precision " << (o.precision == START_PREC_MIXED ? "mixed" : "fp64") << "\n"
+                  << "em_push " << (int)o.emPush << "\n"
+                  << "seed " << o.seed << "\n"
+                  << "density " << (o.densityMode == START_DENSITY_TET_AVERAGE ? "tet_average" : "barycentric") << "\n"
+                  << "sort_interval " << o.sortInterval << "\n"
+                  << "write_back " << (int)o.writeBackEachStep << "\n"
+                  << "movement_max_particles " << o.movementMaxParticles << "\n";
+        for (const auto& [t, q] : o.chargeOverride) std::cout << "charge " << (int)t << " " << q << "\n";
+        for (const auto& [t, d] : o.diameterOverride) std::cout << "diameter " << (int)t << " " << d << "\n";
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "ERROR: " << e.what() << std::endl;
+        return 1;
+    }
+}
+
 int main(int argc, char** argv)
 {
+    if (argc == 3 && std::string(argv[1]) == "--print-config") return print_config(argv[2]);
     if (argc != 2) {
-        std::cerr << "usage: shim_check <dir>" << std::endl;
+        std::cerr << "usage: shim_check <dir> | shim_check --print-config <file.json>" << std::endl;
         return 2;
     }
     std::string dir = argv[1];

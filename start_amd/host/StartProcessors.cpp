@@ -271,14 +271,21 @@ void ParticleKernelDevice::appendMovements(ParticleMovementMap& map, std::mutex&
     check(start_read_movements(m_ctx, &n, ids.data(), counts.data(), xyz.data()), "movements");
     check(start_clear_movements(m_ctx), "movement reset");
     std::lock_guard<std::mutex> lock(mutex);  // ParticleMovementTracker.cpp:10
+    // Slots come in load order, which is ascending id order for a population
+    // loaded with ascending ids: the position after the previous particle's
+    // entry is then the right hint, and each lookup costs O(1) on average
+    // (any other order still finds or inserts the right entry).
+    auto hint = map.begin();
     for (int64_t s = 0; s < n; ++s) {
         const int32_t k = std::min<int32_t>(counts[(size_t)s], 2);
         if (k == 0) continue;
-        auto& pts = map[(size_t)ids[(size_t)s]];
+        auto it = map.try_emplace(hint, (size_t)ids[(size_t)s]);
+        auto& pts = it->second;
         for (int32_t j = 0; j < k; ++j) {
             const double* q = &xyz[((size_t)s * 2 + (size_t)j) * 3];
             pts.push_back({q[0], q[1], q[2]});
         }
+        hint = std::next(it);
     }
 }
 
@@ -320,13 +327,16 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
         check(start_gather_charge(m_ctx), "charge scatter");
         m_charge_fresh = true;
     }
-    check(start_node_charge_density(m_ctx, m_opt.densityMode, m_rho.data(), m_written.data()), "node charge density");
+    // barycentric: every node is written, so the written flags are not read
+    const bool all = m_opt.densityMode == START_DENSITY_BARYCENTRIC;
+    check(start_node_charge_density(m_ctx, m_opt.densityMode, m_rho.data(), all ? nullptr : m_written.data()),
+          "node charge density");
     // One ordered pass: the driver keeps one map for the whole run and never
     // clears it (ModelingMainDriver.cpp:122), so after the first step every
     // key is already there and is only updated; keys of other nodes are kept.
     auto it = out.begin();
     for (size_t r : m_rowsByTag) {
-        if (!m_written[r]) continue;
+        if (!all && !m_written[r]) continue;
         const size_t tag = m_mesh.nodeTags[r];
         while (it != out.end() && it->first < tag) ++it;
         if (it != out.end() && it->first == tag) {

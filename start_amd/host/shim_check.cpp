@@ -20,6 +20,8 @@
 //                 optional movements N (particles tracked in the movement map,
 //                 200,000 by default as ParticleMovementTracker.hpp:43; 0: none),
 //                 timing_warmup W (steps before the timed window)
+//   config.json   optional: the run's JSON configuration, passed to gather and
+//                 process as config_filename (its "Device ..." keys apply)
 // Outputs in <dir>:
 //   out_pos.f64 out_vel.f64  particle state after the loop (ParticleVector order)
 //   out_settled.u64           settled ids, ascending
@@ -91,6 +93,10 @@ int main(int argc, char** argv)
         return 2;
     }
     std::string dir = argv[1];
+    // the run's JSON configuration handed to gather and process as the
+    // reference's loop does (its "Device ..." keys apply over config.txt);
+    // none when <dir>/config.json is absent
+    const std::string run_config = std::ifstream(dir + "/config.json").good() ? dir + "/config.json" : "";
     try {
         std::map<std::string, std::string> cfg;
         {
@@ -178,9 +184,9 @@ int main(int argc, char** argv)
                                            svel[3 * spawned + 1], svel[3 * spawned + 2]);
             if (k == shrink_at) particles.pop_back();
             const clk::time_point a = clk::now();
-            NodeChargeDensityProcessor::gather(t, dir + "/config.txt", device, particles, settled, tracker, density);
+            NodeChargeDensityProcessor::gather(t, run_config, device, particles, settled, tracker, density);
             const clk::time_point b = clk::now();
-            ParticleDynamicsProcessor::process(dir + "/config.txt", particles, t, dt, 16, device, tracker, settled,
+            ParticleDynamicsProcessor::process(run_config, particles, t, dt, 16, device, tracker, settled,
                                                sh, mx, movements, stop, cfg.at("model"), cfg.at("gas"), conc);
             const clk::time_point e = clk::now();
             t_gather += std::chrono::duration<double>(b - a).count();
@@ -192,7 +198,7 @@ int main(int argc, char** argv)
                                                << "\ngather_s " << t_gather << "\nprocess_s " << t_process
                                                << "\nparticle_steps " << (processed1 - processed0) << "\n";
         density.clear();
-        NodeChargeDensityProcessor::gather(t, dir + "/config.txt", device, particles, settled, tracker, density);
+        NodeChargeDensityProcessor::gather(t, run_config, device, particles, settled, tracker, density);
         device.syncParticles(particles);
 
         std::vector<double> op, ov;

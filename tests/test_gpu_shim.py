@@ -3,6 +3,7 @@ NodeChargeDensityProcessor::gather over the C ABI), driven like the reference's
 main loop by shim_check, against the oracle on the same inputs and variates."""
 from __future__ import annotations
 
+import json
 import os
 import subprocess
 
@@ -22,9 +23,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TYPES = ["O2", "Ar", "Ne", "He", "Ti", "Al", "Sn", "W", "Au", "Cu", "Ni", "Ag"]
 
 
-def _run_shim(tmp, w, steps, density, charge_units, diameter, n=None, extra=None, spawn=None):
+def _run_shim(tmp, w, steps, density, charge_units, diameter, n=None, extra=None, spawn=None, run_config=None):
     """shim_check on the first n particles of w; spawn = (pos, vel) of the
-    particles inserted later (extra: spawn_every / spawn_count / shrink_at)."""
+    particles inserted later (extra: spawn_every / spawn_count / shrink_at);
+    run_config: the JSON configuration gather and process are handed."""
     n = len(w.pos) if n is None else n
     tmp.mkdir(exist_ok=True)
     w.mesh.nodes.astype(np.float64).tofile(tmp / "nodes.f64")
@@ -45,6 +47,8 @@ def _run_shim(tmp, w, steps, density, charge_units, diameter, n=None, extra=None
            "temperature": 300.0, "seed": w.seed, "precision": "fp64", "density": density,
            "diameter_Ti": repr(diameter), "charge_Ti": charge_units, **(extra or {})}
     (tmp / "config.txt").write_text("".join(f"{k} {v}\n" for k, v in cfg.items()))
+    if run_config is not None:
+        (tmp / "config.json").write_text(json.dumps(run_config, indent=4))
     log(f"shim_check: {n} particles, {steps} steps, density {density}, {extra or ''}")
     r = subprocess.run([os.path.join(ROOT, "start_amd", "shim_check"), str(tmp)], capture_output=True,
                        text=True, timeout=300)
@@ -173,6 +177,22 @@ def test_cpp_step_loop_with_spawning_matches_oracle(tmp_path, density):
     assert (st.status[:n] == O.SETTLED).any() and (st.status[n:] == O.SETTLED).any()
     log(f"shim spawning {density}: ok, {n_total} particles, settled {ns}, max rel x {ex:.2e} v {ev:.2e} "
         f"rho {eq:.2e}")
+
+
+def test_cpp_step_loop_takes_device_keys_from_the_run_config(tmp_path):
+    """The JSON configuration handed to gather and process (the reference's
+    own keys plus "Device ..." keys) sets the density mode: config.txt asks
+    for the barycentric density, the run config for the tet average, and the
+    loop's map is the tet-average one of the oracle."""
+    charge_units, diameter, steps = 2.0, 2.0e-10, 30
+    w = workloads.small_box(n_particles=1500, charge_units=charge_units)
+    run_config = {"Mesh File": "box.msh", "Time Step": w.dt, "Gas": "Ar", "Model": "HS",
+                  "Device Density Mode": "TetAverage", "Device Precision": "FP64", "Device Seed": w.seed}
+    out = _run_shim(tmp_path / "json", w, steps, "barycentric", charge_units, diameter, run_config=run_config)
+    om, st, sim, q, expect = _oracle_loop(w, steps, charge_units, diameter, len(w.pos))
+    ex, ev, eq, ns = _check(out, w, om, st, sim, q, expect, "tet_average", steps)
+    assert not out["written"].all()  # the tet-average map leaves nodes without a particle unwritten
+    log(f"shim run config: tet-average map from the JSON keys, settled {ns}, rho {eq:.2e}")
 
 
 def test_cpp_step_loop_rejects_a_shrinking_vector(tmp_path):

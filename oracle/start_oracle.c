@@ -687,8 +687,12 @@ void or_barycentric(const or_mesh* m, int32_t t, const double* p, double lam[4])
 
 static void atomic_add_double(double* dst, double val)
 {
+#ifdef OR_PLANT_CHARGE_RACE /* the race-check test's planted mistake: a plain add */
+    *dst += val;
+#else
 #pragma omp atomic
     *dst += val;
+#endif
 }
 
 /* Barycentric charge scatter Q_n += q * lambda_n(x) over the owning tet's nodes

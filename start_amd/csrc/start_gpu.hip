@@ -1852,16 +1852,17 @@ int start_autotune_cached(start_ctx* c, const char* cache_path, int32_t trial_st
     start_launch_cfg b{};
     int rc = start_autotune(c, trial_steps, &b, &ms);
     if (rc != START_OK) return rc;
-    if (FILE* f = fopen(cache_path, "a")) {
-        fprintf(f, "%s|%d %d %d %d %d|%.6g\n", key.c_str(), b.steps_per_launch, b.block, b.persistent, b.sort_interval,
-                b.walk, ms);
-        fclose(f);
-    } else {
-        return fail(c, START_ERR_ARG, std::string("cannot append to the autotune cache ") + cache_path);
-    }
+    // the tuned configuration is set either way; a cache that cannot be
+    // written is reported after the outputs are filled in
     if (best) *best = b;
     if (ms_per_step) *ms_per_step = ms;
     if (from_cache) *from_cache = 0;
+    FILE* f = fopen(cache_path, "a");
+    if (!f) return fail(c, START_ERR_ARG, std::string("cannot append to the autotune cache ") + cache_path);
+    const bool ok = fprintf(f, "%s|%d %d %d %d %d|%.6g\n", key.c_str(), b.steps_per_launch, b.block, b.persistent,
+                            b.sort_interval, b.walk, ms) > 0;
+    if (fclose(f) != 0 || !ok)
+        return fail(c, START_ERR_ARG, std::string("cannot append to the autotune cache ") + cache_path);
     return START_OK;
 }
 

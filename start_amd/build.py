@@ -82,10 +82,10 @@ def build_host(verbose: bool = False) -> list[str]:
     gxx = shutil.which("g++") or "g++"
     common = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), "-I", HOST]
     cmds = [
-        [gxx] + common + ["-fPIC", "-shared", "-o", HOST_LIB + ".tmp", os.path.join(HOST, "StartProcessors.cpp"),
-                          os.path.join(HOST, "DeviceConfig.cpp"), "-L", HERE, "-lstart_gpu", "-Wl,-rpath,$ORIGIN",
-                          "-Wl,--no-undefined"],
-        [gxx] + common + ["-o", SHIM_CHECK + ".tmp", os.path.join(HOST, "shim_check.cpp"),
+        [gxx] + common + ["-fopenmp", "-fPIC", "-shared", "-o", HOST_LIB + ".tmp",
+                          os.path.join(HOST, "StartProcessors.cpp"), os.path.join(HOST, "DeviceConfig.cpp"), "-L", HERE,
+                          "-lstart_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined"],
+        [gxx] + common + ["-fopenmp", "-o", SHIM_CHECK + ".tmp", os.path.join(HOST, "shim_check.cpp"),
                           "-L", HERE, "-lstart_host", "-lstart_gpu", "-Wl,-rpath,$ORIGIN"],
         [gxx] + common + ["-fPIC", "-shared", "-o", IO_LIB + ".tmp", os.path.join(HOST, "GmshReader.cpp"),
                           os.path.join(HOST, "GmshWriter.cpp"), os.path.join(HOST, "Outputs.cpp"),

@@ -2,6 +2,7 @@
 #include "StartProcessors.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cmath>
 #include <iostream>
@@ -271,6 +272,49 @@ void ParticleKernelDevice::appendMovements(ParticleMovementMap& map, std::mutex&
     check(start_read_movements(m_ctx, &n, ids.data(), counts.data(), xyz.data()), "movements");
     check(start_clear_movements(m_ctx), "movement reset");
     std::lock_guard<std::mutex> lock(mutex);  // ParticleMovementTracker.cpp:10
+    // After a particle's first movement its key is in the map.  When every
+    // particle with points this step already has its key, the points are
+    // appended on several threads, each over a contiguous range of slots:
+    // first a read-only pass finds the entries (slots come in load order, so
+    // for ids loaded in ascending order a forward walk from the range's first
+    // key finds each in O(1); otherwise a lookup), then each thread appends to
+    // its own particles' vectors.  Any missing key sends the step to the
+    // serial loop below, which inserts.
+    if (n >= 16384 && !map.empty()) {
+        const int T = 8;
+        std::vector<ParticleMovementMap::iterator> where((size_t)n, map.end());
+        std::atomic<bool> missing{false};
+#pragma omp parallel for num_threads(T) schedule(static)
+        for (int c = 0; c < T; ++c) {
+            const int64_t s0 = n * c / T, s1 = n * (c + 1) / T;
+            auto it = map.lower_bound((size_t)ids[(size_t)s0]);
+            for (int64_t q = s0; q < s1 && !missing.load(std::memory_order_relaxed); ++q) {
+                if (counts[(size_t)q] == 0) continue;
+                const size_t id = (size_t)ids[(size_t)q];
+                while (it != map.end() && it->first < id) ++it;
+                if (it == map.end() || it->first != id) it = map.find(id);
+                if (it == map.end()) {
+                    missing = true;
+                    break;
+                }
+                where[(size_t)q] = it;
+            }
+        }
+        if (!missing) {
+#pragma omp parallel for num_threads(T) schedule(static)
+            for (int c = 0; c < T; ++c) {
+                const int64_t s0 = n * c / T, s1 = n * (c + 1) / T;
+                for (int64_t q = s0; q < s1; ++q) {
+                    const int32_t k = std::min<int32_t>(counts[(size_t)q], 2);
+                    for (int32_t j = 0; j < k; ++j) {
+                        const double* p = &xyz[((size_t)q * 2 + (size_t)j) * 3];
+                        where[(size_t)q]->second.push_back({p[0], p[1], p[2]});
+                    }
+                }
+            }
+            return;
+        }
+    }
     // Slots come in load order, which is ascending id order for a population
     // loaded with ascending ids: the position after the previous particle's
     // entry is then the right hint, and each lookup costs O(1) on average
@@ -331,9 +375,36 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
     const bool all = m_opt.densityMode == START_DENSITY_BARYCENTRIC;
     check(start_node_charge_density(m_ctx, m_opt.densityMode, m_rho.data(), all ? nullptr : m_written.data()),
           "node charge density");
-    // One ordered pass: the driver keeps one map for the whole run and never
-    // clears it (ModelingMainDriver.cpp:122), so after the first step every
-    // key is already there and is only updated; keys of other nodes are kept.
+    // The driver keeps one map for the whole run and never clears it
+    // (ModelingMainDriver.cpp:122), so after the first step every key is
+    // already there and only its value changes.  Then the ordered pass runs in
+    // chunks on several threads: each finds its first key, walks its part of
+    // the map and writes values only (no insertion, so the tree is only read);
+    // a written node without a key sends the whole pass to the serial loop
+    // below, which inserts.  Keys of other nodes are kept.
+    const size_t M = m_rowsByTag.size();
+    if (out.size() >= M / 2 && M >= 16384) {
+        const int T = 8;
+        std::atomic<bool> missing{false};
+#pragma omp parallel for num_threads(T) schedule(static)
+        for (int c = 0; c < T; ++c) {
+            const size_t k0 = M * (size_t)c / T, k1 = M * (size_t)(c + 1) / T;
+            auto it = out.lower_bound(m_mesh.nodeTags[m_rowsByTag[k0]]);
+            for (size_t k = k0; k < k1 && !missing.load(std::memory_order_relaxed); ++k) {
+                const size_t r = m_rowsByTag[k];
+                if (!all && !m_written[r]) continue;
+                const size_t tag = m_mesh.nodeTags[r];
+                while (it != out.end() && it->first < tag) ++it;
+                if (it == out.end() || it->first != tag) {
+                    missing = true;
+                    break;
+                }
+                it->second = m_rho[r];
+                ++it;
+            }
+        }
+        if (!missing) return;
+    }
     auto it = out.begin();
     for (size_t r : m_rowsByTag) {
         if (!all && !m_written[r]) continue;

@@ -179,6 +179,20 @@ def test_cpp_step_loop_with_spawning_matches_oracle(tmp_path, density):
         f"rho {eq:.2e}")
 
 
+@pytest.mark.parametrize("density", ["barycentric", "tet_average"])
+def test_cpp_step_loop_on_a_mesh_with_many_nodes(tmp_path, density):
+    """19,683 nodes: from the second gather on, the density map's values are
+    written by the chunked multi-threaded pass (maps of 16,384 nodes and
+    more), which must leave the same map as the serial one."""
+    charge_units, diameter, steps = 2.0, 2.0e-10, 12
+    w = workloads.large_box(n_particles=30_000, cells=26)
+    assert w.mesh.n_nodes == 27 ** 3
+    out = _run_shim(tmp_path / density, w, steps, density, charge_units, diameter)
+    om, st, sim, q, expect = _oracle_loop(w, steps, charge_units, diameter, len(w.pos))
+    ex, ev, eq, ns = _check(out, w, om, st, sim, q, expect, density, steps)
+    log(f"shim {density} on 19,683 nodes: ok, written {int(out['written'].sum())}, rho {eq:.2e}")
+
+
 def test_cpp_step_loop_takes_device_keys_from_the_run_config(tmp_path):
     """The JSON configuration handed to gather and process (the reference's
     own keys plus "Device ..." keys) sets the density mode: config.txt asks

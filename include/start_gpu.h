@@ -395,6 +395,14 @@ int start_mesh_maps(const start_mesh_desc* mesh, int32_t* tets_out, double* geo,
                     double* tau, float* geo32, double* tau32);
 /* Device exact orientation signs of n quadruples (12 doubles each), for checks. */
 int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign);
+/* The device's FP64 elementary functions of the collision path and the
+ * particle sources, over n arguments (host arrays), for checks against the
+ * oracle's: which = 0 pow(x, y), 1 sin x (out) and cos x (out2), 2 acos x,
+ * 3 atan2(y, x), 4 log x, 5 exp x.  out2 is written (zeros) for every which.
+ * No counterpart in the reference, which calls the C library's functions
+ * (HSModel.cpp:12-34, VSSModel.cpp:32-46, GeometryVector.hpp:448-474,
+ * ParticleGeneratorHost.cpp:130-146). */
+int start_math_batch(int32_t which, int64_t n, const double* x, const double* y, double* out, double* out2);
 
 /* Timing hooks: HIP events recorded around the step kernels on the library's
  * stream; returns the summed kernel milliseconds since the last reset. */

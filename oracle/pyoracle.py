@@ -59,6 +59,9 @@ def lib() -> C.CDLL:
         L.or_seg_tri_intersect.restype = C.c_int
         L.or_gas_concentration.argtypes = [d, d]
         L.or_gas_concentration.restype = d
+        L.or_set_libm.argtypes = [C.c_int]
+        L.or_get_libm.restype = C.c_int
+        L.or_math_eval.argtypes = [C.c_int, C.c_int, i64, P(d), P(d), P(d), P(d)]
         L.or_species_constants.argtypes = [P(SpeciesDesc), P(GasDesc), P(SpeciesK)]
         L.or_surface_bbox.argtypes = [i64, P(d), i64, P(i32), d, P(d), P(d)]
         L.or_collide.argtypes = [C.c_int, P(SpeciesK), d, d, P(d), d, d, d]
@@ -127,6 +130,30 @@ def seg_tri(p, q, a, b, c):
     t = C.c_double(-1.0)
     hit = lib().or_seg_tri_intersect(*[_p(x, C.c_double) for x in arr], C.byref(t))
     return bool(hit), t.value
+
+
+def set_libm(on: bool) -> None:
+    """Collision-path elementary functions: the C library's (True) or the
+    shared deterministic ones of start_amd/csrc/detmath.h (False, the default)."""
+    lib().or_set_libm(int(bool(on)))
+
+
+def get_libm() -> bool:
+    return bool(lib().or_get_libm())
+
+
+MATH_FN = {"pow": 0, "sincos": 1, "acos": 2, "atan2": 3, "log": 4, "exp": 5}
+
+
+def math_eval(fn: str, x, y=None, libm: bool = False):
+    """fn over arrays: pow(x, y), sincos(x) -> (sin, cos), acos(x), atan2(y, x), log(x), exp(x)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64)
+    assert x.shape == y.shape and x.ndim == 1
+    out, out2 = np.empty_like(x), np.empty_like(x)
+    lib().or_math_eval(MATH_FN[fn], int(libm), len(x), _p(x, C.c_double), _p(y, C.c_double),
+                       _p(out, C.c_double), _p(out2, C.c_double))
+    return (out, out2) if fn == "sincos" else out
 
 
 def gas_concentration(P: float, T: float) -> float:

@@ -20,9 +20,47 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import pyoracle as _O
+
 EV_J = 1.60217733000001e-19  # Constants.hpp:15
 PI = 3.14159265358979323846
 M32 = np.uint64(0xFFFFFFFF)
+
+
+# sin, cos, acos, atan2, log and exp through the C oracle: the shared
+# deterministic functions the device also compiles, or the C library's in the
+# oracle's libm mode (pyoracle.set_libm)
+def _fn(name, x, y=None):
+    scalar = np.ndim(x) == 0
+    xa = np.atleast_1d(np.asarray(x, dtype=np.float64))
+    ya = None if y is None else np.broadcast_to(np.asarray(y, dtype=np.float64), xa.shape)
+    r = _O.math_eval(name, xa, ya, libm=_O.get_libm())
+    return (r[0][0] if scalar else r[0], r[1][0] if scalar else r[1]) if name == "sincos" else \
+        (r[0] if scalar else r)
+
+
+def _sin(x):
+    return _fn("sincos", x)[0]
+
+
+def _cos(x):
+    return _fn("sincos", x)[1]
+
+
+def _log(x):
+    return _fn("log", x)
+
+
+def _exp(x):
+    return _fn("exp", x)
+
+
+def _acos(x):
+    return _fn("acos", x)
+
+
+def _atan2(y, x):
+    return _fn("atan2", x, y)
 
 
 def philox(c0, c1, c2, c3, k0, k1):
@@ -80,8 +118,8 @@ def generate(sources, masses, seed):
             pos = np.tile(np.asarray(src.get("position", (0.0, 0.0, 0.0)), dtype=np.float64), (n, 1))
             t0 = np.full(n, float(src.get("theta", 0.0)))
             p0 = np.full(n, float(src.get("phi", 0.0)))
-            st = np.sin(t0)
-            d = np.stack([st * np.cos(p0), st * np.sin(p0), np.cos(t0)], axis=1)
+            st = _sin(t0)
+            d = np.stack([st * _cos(p0), st * _sin(p0), _cos(t0)], axis=1)
         else:
             cen = np.asarray(src["centres"], dtype=np.float64).reshape(-1, 3)
             nor = np.asarray(src["normals"], dtype=np.float64).reshape(-1, 3)
@@ -94,8 +132,8 @@ def generate(sources, masses, seed):
             pos = cen[cell].copy()
             nn = nor[cell]
             mag = np.sqrt(nn[:, 0] * nn[:, 0] + nn[:, 1] * nn[:, 1] + nn[:, 2] * nn[:, 2])
-            t0 = np.arccos(nn[:, 2] / mag)
-            p0 = np.arctan2(nn[:, 1] / mag, nn[:, 0] / mag)
+            t0 = _acos(nn[:, 2] / mag)
+            p0 = _atan2(nn[:, 1] / mag, nn[:, 0] / mag)
             d = nn.copy()
         if disp != 0.0:  # ParticleDisplacer.cpp:20-31
             mc = np.max(np.abs(d), axis=1)
@@ -104,9 +142,9 @@ def generate(sources, masses, seed):
         energy = src.get("energy", "mono")
         if energy == "maxwellian":
             sig = np.sqrt((float(src["p0"]) * EV_J) / m)
-            r1 = np.sqrt(-2.0 * np.log(1.0 - u0))
-            vel = np.stack([sig * (r1 * np.cos(2.0 * PI * u1)), sig * (r1 * np.sin(2.0 * PI * u1)),
-                            sig * (np.sqrt(-2.0 * np.log(1.0 - u2)) * np.cos(2.0 * PI * u3))], axis=1)
+            r1 = np.sqrt(-2.0 * _log(1.0 - u0))
+            vel = np.stack([sig * (r1 * _cos(2.0 * PI * u1)), sig * (r1 * _sin(2.0 * PI * u1)),
+                            sig * (np.sqrt(-2.0 * _log(1.0 - u2)) * _cos(2.0 * PI * u3))], axis=1)
         else:
             if energy == "mono":
                 e = np.full(n, float(src["energy_eV"]))
@@ -117,28 +155,28 @@ def generate(sources, masses, seed):
                 e = U * rr / (1.0 - rr)
             else:
                 mean, sl = float(src["p0"]), float(src["p1"])
-                mu = np.log(mean) - 0.5 * sl * sl
-                e = np.exp(mu + sl * (np.sqrt(-2.0 * np.log(1.0 - u0)) * np.cos(2.0 * PI * u1)))
+                mu = _log(mean) - 0.5 * sl * sl
+                e = _exp(mu + sl * (np.sqrt(-2.0 * _log(1.0 - u0)) * _cos(2.0 * PI * u1)))
             v = np.sqrt(2 * (e * EV_J) / m)
             angle = src.get("angle", "fan")
             if angle == "fan":
                 a = float(src.get("expansion", 0.0))
                 th = t0 + (-1.0 + 2.0 * u2) * a if a != 0.0 else t0
-                vel = np.stack([v * np.sin(th) * np.cos(p0), v * np.sin(th) * np.sin(p0), v * np.cos(th)], axis=1)
+                vel = np.stack([v * _sin(th) * _cos(p0), v * _sin(th) * _sin(p0), v * _cos(th)], axis=1)
             else:
                 if angle == "cosine":
                     ct = np.sqrt(u2)
                 elif angle == "isotropic":
                     ct = 1.0 - 2.0 * u2
                 else:
-                    ct = 1.0 - u2 * (1.0 - np.cos(float(src.get("expansion", 0.0))))
+                    ct = 1.0 - u2 * (1.0 - _cos(float(src.get("expansion", 0.0))))
                 ph = 2.0 * PI * u3
                 stt = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
-                a_, b_ = stt * np.cos(ph), stt * np.sin(ph)
+                a_, b_ = stt * _cos(ph), stt * _sin(ph)
                 if angle == "isotropic":
                     vel = np.stack([v * a_, v * b_, v * ct], axis=1)
                 else:
-                    s0, c0, sp, cp = np.sin(t0), np.cos(t0), np.sin(p0), np.cos(p0)
+                    s0, c0, sp, cp = _sin(t0), _cos(t0), _sin(p0), _cos(p0)
                     vel = np.stack([v * ((a_ * (c0 * cp) - b_ * sp) + ct * (s0 * cp)),
                                     v * ((a_ * (c0 * sp) + b_ * cp) + ct * (s0 * sp)),
                                     v * (ct * c0 - a_ * s0)], axis=1)

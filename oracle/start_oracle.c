@@ -41,6 +41,65 @@
 #define OR_PI 3.14159265358979323846 /* START_PI_NUMBER, PreprocessorUtils.hpp:73-77 */
 
 /* ------------------------------------------------------------------------- */
+/* Elementary functions of the collision path.  The reference calls the C    */
+/* library's pow, sin, cos, acos and atan2 (HSModel.cpp:12-34,               */
+/* VHSModel.cpp:19-43, VSSModel.cpp:32-46, GeometryVector.hpp:448-474).  The */
+/* oracle calls by default the deterministic ones of                         */
+/* start_amd/csrc/detmath.h, which the HIP kernels compile from the same     */
+/* source, so that GPU and oracle agree to the bit; or_set_libm(1) switches  */
+/* to the C library's.  detmath.h is pinned against the C library by         */
+/* tests/test_detmath.py (within 2 ulps), and the two oracle modes against   */
+/* each other on whole runs.                                                 */
+/* ------------------------------------------------------------------------- */
+#define DM_FN static inline
+#define DM_TABLE static const
+#include "detmath.h"
+
+static int or_libm = 0;
+void or_set_libm(int on) { or_libm = on; }
+int or_get_libm(void) { return or_libm; }
+static double or_pow(double x, double y) { return or_libm ? pow(x, y) : dm_pow(x, y); }
+static double or_sin(double x)
+{
+    double s, c;
+    if (or_libm) return sin(x);
+    dm_sincos(x, &s, &c);
+    return s;
+}
+static double or_cos(double x)
+{
+    double s, c;
+    if (or_libm) return cos(x);
+    dm_sincos(x, &s, &c);
+    return c;
+}
+static double or_acos(double x) { return or_libm ? acos(x) : dm_acos(x); }
+static double or_atan2(double y, double x) { return or_libm ? atan2(y, x) : dm_atan2(y, x); }
+static double or_log(double x) { return or_libm ? log(x) : dm_log(x); }
+static double or_exp(double x) { return or_libm ? exp(x) : dm_exp(x); }
+
+/* The selected functions (which: 0 pow(x, y), 1 sin x and cos x, 2 acos x,
+ * 3 atan2(y, x), 4 log x, 5 exp x) of the shared code (libm = 0) or of the C
+ * library (libm = 1) over n arguments: for the tests and for the particle
+ * sources' restatement (oracle/pysources.py). */
+void or_math_eval(int which, int libm, int64_t n, const double* x, const double* y, double* out, double* out2)
+{
+    const int saved = or_libm;
+    or_libm = libm;
+    for (int64_t i = 0; i < n; ++i) {
+        switch (which) {
+        case 0: out[i] = or_pow(x[i], y[i]); break;
+        case 1: out[i] = or_sin(x[i]); out2[i] = or_cos(x[i]); break;
+        case 2: out[i] = or_acos(x[i]); break;
+        case 3: out[i] = or_atan2(y[i], x[i]); break;
+        case 4: out[i] = or_log(x[i]); break;
+        default: out[i] = or_exp(x[i]); break;
+        }
+    }
+    or_libm = saved;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as   */
 /* easy as 1, 2, 3", SC'11).  Replaces the reference's per-call              */
 /* random_device-seeded mt19937 (HSModel.cpp:18, RealNumberGeneratorHost.cpp:14) */
@@ -422,7 +481,7 @@ int or_collide(int model, const or_species_k* k, double n, double dt, double v[3
     if (model == OR_MODEL_HS) {
         sigma = k->sigma_hs;
     } else {
-        double d_vhs_2 = k->vhs_c * pow(k->two_kt / (k->m_r * vmod * vmod), k->vhs_exp);
+        double d_vhs_2 = k->vhs_c * or_pow(k->two_kt / (k->m_r * vmod * vmod), k->vhs_exp);
         sigma = OR_PI * d_vhs_2;
     }
     double P = sigma * vmod * n * dt;
@@ -430,22 +489,22 @@ int or_collide(int model, const or_species_k* k, double n, double dt, double v[3
     double xi_cos;
     if (model == OR_MODEL_HS) xi_cos = (u2 * (1.0 - (-1.0))) + (-1.0);
     else if (model == OR_MODEL_VHS) xi_cos = (u2 * (1.0 - 0.0)) + 0.0;
-    else xi_cos = 2 * pow(u2, k->inv_alpha) - 1.;
+    else xi_cos = 2 * or_pow(u2, k->inv_alpha) - 1.;
     double xi_sin = sqrt(1 - xi_cos * xi_cos);
     double phi = (u3 * (2 * OR_PI - 0)) + 0;
-    double x = xi_sin * cos(phi), y = xi_sin * sin(phi), z = xi_cos;
+    double x = xi_sin * or_cos(phi), y = xi_sin * or_sin(phi), z = xi_cos;
     double cm[3] = {v[0] * k->mass_cp, v[1] * k->mass_cp, v[2] * k->mass_cp};
     double pv[3] = {v[0] * k->mass_ct, v[1] * k->mass_ct, v[2] * k->mass_ct};
     double mp = sqrt(pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]);
     double dir[3] = {x * mp, y * mp, z * mp};
     if (model == OR_MODEL_VSS) {
         /* GeometryVector::calcBetaGamma + rotation (GeometryVector.hpp:448-474, :66-90) */
-        double beta = acos(pv[2] / mp), gamma = atan2(pv[1], pv[0]);
-        double cb = cos(beta), sb = sin(beta);
+        double beta = or_acos(pv[2] / mp), gamma = or_atan2(pv[1], pv[0]);
+        double cb = or_cos(beta), sb = or_sin(beta);
         double tx = cb * dir[0] + sb * dir[2];
         double tz = -sb * dir[0] + cb * dir[2];
         dir[0] = tx; dir[2] = tz;
-        double cg = cos(gamma), sg = sin(gamma);
+        double cg = or_cos(gamma), sg = or_sin(gamma);
         tx = cg * dir[0] - sg * dir[1];
         double ty = sg * dir[0] + cg * dir[1];
         dir[0] = tx; dir[1] = ty;

@@ -87,6 +87,15 @@ int or_collide(int model, const or_species_k* k, double concentration, double dt
                double u1, double u2, double u3);
 void or_push(const double E[3], double q, double m, double dt, double v[3]);
 
+/* Elementary functions of the collision path: 0 (default) the shared
+ * deterministic ones the HIP kernels also compile (start_amd/csrc/detmath.h),
+ * 1 the C library's, as the reference calls them.  Set before stepping. */
+void or_set_libm(int on);
+int or_get_libm(void);
+/* which = 0 pow(x, y), 1 sin x (out) and cos x (out2), 2 acos x,
+ * 3 atan2(y, x), 4 log x, 5 exp x, over n arguments, in the given mode. */
+void or_math_eval(int which, int libm, int64_t n, const double* x, const double* y, double* out, double* out2);
+
 /* ---- mesh ---- */
 or_mesh* or_mesh_build(int64_t nnodes, const double* nodes, int64_t ntets, const int32_t* tets,
                        int64_t ntris, const int32_t* tris);

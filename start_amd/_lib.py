@@ -23,7 +23,7 @@ EXPORTS = [
     "start_load_particles", "start_set_tet_field", "start_set_node_potential", "start_step",
     "start_gather_charge", "start_read_node_charge", "start_read_particles",
     "start_read_tri_counts", "start_read_settled", "start_read_stats", "start_synchronize",
-    "start_locate", "start_orient3d_batch", "start_timing_enable", "start_timing_read",
+    "start_locate", "start_orient3d_batch", "start_math_batch", "start_timing_enable", "start_timing_read",
     "start_node_charge_density", "start_set_gas", "start_set_time_step",
     "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
     L.start_locate.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
     L.start_locate_noalloc.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
     L.start_orient3d_batch.argtypes = [i64, P(d), P(i32)]
+    L.start_math_batch.argtypes = [i32, i64, P(d), P(d), P(d), P(d)]
     L.start_mesh_maps.argtypes = [P(MeshDesc), P(i32), P(d), P(d), P(d), P(C.c_float), P(d)]
     L.start_timing_enable.argtypes = [vp, i32]
     L.start_timing_read.argtypes = [vp, P(d), P(u64)]

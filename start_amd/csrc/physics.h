@@ -15,6 +15,12 @@
 #include <math.h>
 #include <stdint.h>
 
+// pow, sin, cos, acos and atan2 of the FP64 collision path, the same code as
+// the oracle's (dm_* functions)
+#define DM_FN __device__ inline
+#define DM_TABLE static __constant__ const
+#include "detmath.h"
+
 namespace sg {
 
 constexpr double kPi = 3.14159265358979323846;  // START_PI_NUMBER
@@ -137,27 +143,9 @@ __device__ inline float m_nextafter(float x, float y) { return nextafterf(x, y);
 template <typename R>
 __device__ inline R m_pow(R x, R y);
 template <>
-__device__ inline double m_pow<double>(double x, double y) { return pow(x, y); }
+__device__ inline double m_pow<double>(double x, double y) { return dm_pow(x, y); }
 template <>
 __device__ inline float m_pow<float>(float x, float y) { return powf(x, y); }
-template <typename R>
-__device__ inline void m_sincos(R x, R* s, R* c);
-template <>
-__device__ inline void m_sincos<double>(double x, double* s, double* c) { *s = sin(x); *c = cos(x); }
-template <>
-__device__ inline void m_sincos<float>(float x, float* s, float* c) { *s = sinf(x); *c = cosf(x); }
-template <typename R>
-__device__ inline R m_acos(R x);
-template <>
-__device__ inline double m_acos<double>(double x) { return acos(x); }
-template <>
-__device__ inline float m_acos<float>(float x) { return acosf(x); }
-template <typename R>
-__device__ inline R m_atan2(R y, R x);
-template <>
-__device__ inline double m_atan2<double>(double y, double x) { return atan2(y, x); }
-template <>
-__device__ inline float m_atan2<float>(float y, float x) { return atan2f(y, x); }
 
 // Collision probability P = ((sigma*|v|)*n)*dt (HSModel.cpp:15, VHSModel.cpp:24-25).
 template <typename R>
@@ -186,7 +174,7 @@ __device__ inline void scatter_velocity(int model, double mass_cp, double mass_c
     double xi_sin = m_sqrt<double>(1.0 - xi_cos * xi_cos);
     double phi = (u3 * (2.0 * kPi - 0.0)) + 0.0;                      // rng(0, 2*PI)
     double sphi, cphi;
-    m_sincos<double>(phi, &sphi, &cphi);
+    dm_sincos(phi, &sphi, &cphi);
     double x = xi_sin * cphi, y = xi_sin * sphi, z = xi_cos;
     double cmx = vx * mass_cp, cmy = vy * mass_cp, cmz = vz * mass_cp;
     double px = vx * mass_ct, py = vy * mass_ct, pz = vz * mass_ct;
@@ -195,9 +183,9 @@ __device__ inline void scatter_velocity(int model, double mass_cp, double mass_c
     if (model == 3) {
         // calcBetaGamma + rotation (GeometryVector.hpp:448-474, :66-90)
         double sb, cb, sgm, cg;
-        double beta = m_acos<double>(pz / mp), gamma = m_atan2<double>(py, px);
-        m_sincos<double>(beta, &sb, &cb);
-        m_sincos<double>(gamma, &sgm, &cg);
+        double beta = dm_acos(pz / mp), gamma = dm_atan2(py, px);
+        dm_sincos(beta, &sb, &cb);
+        dm_sincos(gamma, &sgm, &cg);
         double tx = cb * dx + sb * dz;
         double tz = -sb * dx + cb * dz;
         dx = tx;

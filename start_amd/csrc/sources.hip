@@ -2,7 +2,9 @@
 // point and surface sources (ParticleGeneratorHost.cpp:33-159,
 // ParticleGeneratorDevice.cu:56-304), its start-point displacement
 // (ParticleDisplacer.cpp:20-31, 48-134) and the energy / direction samplers
-// the workloads need, drawn straight into the context's FP64 arrays.
+// the workloads need, drawn straight into the context's FP64 arrays.  Their
+// sin, cos, acos, atan2, log and exp are the shared deterministic ones of
+// detmath.h, which the oracle's restatement calls too.
 #include "kernels.h"
 #include "sources.h"
 
@@ -27,7 +29,7 @@ __device__ inline void src_uniforms(uint64_t seed, uint32_t i, uint32_t s, uint3
 // standard normal from two uniforms (Box-Muller, cosine branch)
 __device__ inline double normal_bm(double ua, double ub)
 {
-    return sqrt(-2.0 * log(1.0 - ua)) * cos(2.0 * kPi * ub);
+    return sqrt(-2.0 * dm_log(1.0 - ua)) * dm_cos(2.0 * kPi * ub);
 }
 
 __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, const double* __restrict__ centres,
@@ -51,10 +53,10 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
         t0 = S.theta;
         p0 = S.phi;
         // ParticleDisplacer.cpp:68-71
-        const double st = sin(t0);
-        dx = st * cos(p0);
-        dy = st * sin(p0);
-        dz = cos(t0);
+        const double st = dm_sin(t0);
+        dx = st * dm_cos(p0);
+        dy = st * dm_sin(p0);
+        dz = dm_cos(t0);
     } else {
         // count / n_cells particles per cell, one more for the first count % n_cells
         // cells of the (host-shuffled) cell order
@@ -68,8 +70,8 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
         // ParticleGeneratorHost.cpp:130-146: normalised normal, theta = acos(n_z), phi = atan2(n_y, n_x)
         const double mag = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
         const double nx = nn[0] / mag, ny = nn[1] / mag, nz = nn[2] / mag;
-        t0 = acos(nz);
-        p0 = atan2(ny, nx);
+        t0 = dm_acos(nz);
+        p0 = dm_atan2(ny, nx);
         dx = nn[0];  // the displacer takes the normal as given (ParticleDisplacer.cpp:110-113)
         dy = nn[1];
         dz = nn[2];
@@ -88,7 +90,7 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
         // each component normal with variance kT/m
         const double sig = sqrt((S.p0 * kEvJ) / S.mass);
         wx = sig * normal_bm(u0, u1);
-        wy = sig * (sqrt(-2.0 * log(1.0 - u0)) * sin(2.0 * kPi * u1));
+        wy = sig * (sqrt(-2.0 * dm_log(1.0 - u0)) * dm_sin(2.0 * kPi * u1));
         wz = sig * normal_bm(u2, u3);
     } else {
         double e;  // eV
@@ -100,8 +102,8 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
             const double rr = sqrt(u0 * (fr * fr));
             e = S.p0 * rr / (1.0 - rr);
         } else {  // lognormal with mean p0 and log-standard deviation p1
-            const double mu = log(S.p0) - 0.5 * S.p1 * S.p1;
-            e = exp(mu + S.p1 * normal_bm(u0, u1));
+            const double mu = dm_log(S.p0) - 0.5 * S.p1 * S.p1;
+            e = dm_exp(mu + S.p1 * normal_bm(u0, u1));
         }
         // ParticleDynamicUtils.hpp:52-96: E in J, v = sqrt(2 E / m)
         const double ej = e * kEvJ;
@@ -109,9 +111,9 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
         if (S.angle == START_ANGLE_FAN) {
             // polar angle theta + u * expansion (u uniform in [-1, 1)), azimuth phi
             const double th = S.expansion != 0.0 ? t0 + (-1.0 + 2.0 * u2) * S.expansion : t0;
-            wx = v * sin(th) * cos(p0);
-            wy = v * sin(th) * sin(p0);
-            wz = v * cos(th);
+            wx = v * dm_sin(th) * dm_cos(p0);
+            wy = v * dm_sin(th) * dm_sin(p0);
+            wz = v * dm_cos(th);
         } else {
             double ct, ph;
             if (S.angle == START_ANGLE_COSINE) {
@@ -119,11 +121,11 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
             } else if (S.angle == START_ANGLE_ISOTROPIC) {
                 ct = 1.0 - 2.0 * u2;
             } else {  // cone of half-angle expansion
-                ct = 1.0 - u2 * (1.0 - cos(S.expansion));
+                ct = 1.0 - u2 * (1.0 - dm_cos(S.expansion));
             }
             ph = 2.0 * kPi * u3;
             const double st = sqrt(fmax(0.0, 1.0 - ct * ct));
-            const double a = st * cos(ph), b = st * sin(ph);
+            const double a = st * dm_cos(ph), b = st * dm_sin(ph);
             if (S.angle == START_ANGLE_ISOTROPIC) {
                 wx = v * a;
                 wy = v * b;
@@ -131,7 +133,7 @@ __global__ void generate_kernel(SourceK S, uint32_t s_index, uint64_t seed, cons
             } else {
                 // local (a, b, ct) about d = (sin t0 cos p0, sin t0 sin p0, cos t0):
                 // e1 = (cos t0 cos p0, cos t0 sin p0, -sin t0), e2 = (-sin p0, cos p0, 0)
-                const double s0 = sin(t0), c0 = cos(t0), sp = sin(p0), cp = cos(p0);
+                const double s0 = dm_sin(t0), c0 = dm_cos(t0), sp = dm_sin(p0), cp = dm_cos(p0);
                 wx = v * ((a * (c0 * cp) - b * sp) + ct * (s0 * cp));
                 wy = v * ((a * (c0 * sp) + b * cp) + ct * (s0 * sp));
                 wz = v * (ct * c0 - a * s0);

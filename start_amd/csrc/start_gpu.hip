@@ -48,6 +48,8 @@ void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt);
 void launch_push_table(hipStream_t s, const double* E, const float* E32, int64_t nt, const double* q,
                        const double* m, int32_t n_slots, double dt, double* H, float* H32);
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out);
+void launch_math_batch(hipStream_t s, int which, const double* x, const double* y, int64_t n, double* out,
+                       double* out2);
 void launch_abs_positions(hipStream_t s, const DevMesh& M, const float* lx, const float* ly, const float* lz,
                           const int32_t* tet, int64_t n, double* x, double* y, double* z);
 size_t sort_temp_bytes(int64_t n, int nbits);
@@ -2304,6 +2306,26 @@ int start_orient3d_batch(int64_t n, const double* pts, int32_t* out_sign)
     (void)hipFree(dp);
     (void)hipFree(ds);
     if (e != hipSuccess) return fail(nullptr, START_ERR_HIP, std::string("orient3d batch: ") + hipGetErrorString(e));
+    return START_OK;
+}
+
+int start_math_batch(int32_t which, int64_t n, const double* x, const double* y, double* out, double* out2)
+{
+    if (which < 0 || which > 5) return fail(nullptr, START_ERR_ARG, "which must be in [0, 5]");
+    if (n < 0 || (n > 0 && (!x || !y || !out || !out2))) return fail(nullptr, START_ERR_ARG, "bad argument");
+    if (n == 0) return START_OK;
+    double* d = nullptr;
+    hipError_t e = dalloc(&d, 4 * n);
+    if (e == hipSuccess) e = hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, y, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_math_batch(nullptr, which, d, d + n, n, d + 2 * n, d + 3 * n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out2, d + 3 * n, sizeof(double) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(nullptr, START_ERR_HIP, std::string("math batch: ") + hipGetErrorString(e));
     return START_OK;
 }
 

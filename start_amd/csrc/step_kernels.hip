@@ -919,10 +919,30 @@ struct StepResult {
     int32_t hit_tri;
 };
 
+// Post-collision velocity of a particle whose collision test passed in this
+// step, with the FP64 expressions in both precisions (rare: about once per 1e4
+// particle-steps).  Evaluated in FP32, sin(chi) = sqrt((1 - cos)(1 + cos)) of
+// a forward scatter (cos near 1 from an FP64 variate rounded to FP32) loses
+// most of its digits.
+template <typename R>
+__device__ __forceinline__ void scatter_after(const StepParams& S, const SpeciesK& k, const RunK* __restrict__ rk,
+                                              uint32_t vid, uint32_t step, PState<R>& s)
+{
+    double u2, u3;
+    variates(S.seed, rk->key, vid, step, 1u, u2, u3);
+    V3 nv = collide_velocity(S.model, k.mass_cp, k.mass_ct, k.inv_alpha, u2, u3, s.vx, s.vy, s.vz);
+    s.vx = nv.x;
+    s.vy = nv.y;
+    s.vz = nv.z;
+}
+
+// The post-collision velocity is applied by the caller once step_one has
+// returned (scatter_after): nothing in the step after the collision test reads
+// v, and there few values are live across the out-of-line call.
 template <typename R, bool kBatch, bool kWalk, typename Sink>
 __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParams& S, const SpeciesK& k,
                                                const RunK* __restrict__ rk, uint32_t vid, uint32_t step,
-                                               int remaining, PState<R>& s, Sink& sink)
+                                               int remaining, PState<R>& s, Sink& sink, bool& scatter)
 {
     const double tau = tau_of<R>(M);
     // 2. |v| < 10 m/s and 3. outside the expanded bbox: frozen this step and
@@ -988,24 +1008,12 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
         }
         return StepResult{kDone, -1};
     }
-    // 7. gas collision (ParticlePhysicsUpdater.cpp:20-24)
+    // 7. gas collision test (ParticlePhysicsUpdater.cpp:20-24); the new
+    //    velocity is scatter_after's
     if (S.model != 0) {
         double u1, u1b;
-        const uint32_t key = rk->key;
-        variates(S.seed, key, vid, step, 0u, u1, u1b);
-        if (collides<R>(S, k, rk->conc, (R)s.vx, (R)s.vy, (R)s.vz, u1)) {
-            double u2, u3;
-            variates(S.seed, key, vid, step, 1u, u2, u3);
-            // Post-collision velocity with the FP64 expressions in both
-            // precisions (rare: about once per 1e4 particle-steps).  Evaluated
-            // in FP32, sin(chi) = sqrt((1 - cos)(1 + cos)) of a forward scatter
-            // (cos near 1 from an FP64 variate rounded to FP32) loses most of
-            // its digits.
-            V3 nv = collide_velocity(S.model, k.mass_cp, k.mass_ct, k.inv_alpha, u2, u3, s.vx, s.vy, s.vz);
-            s.vx = nv.x;
-            s.vy = nv.y;
-            s.vz = nv.z;
-        }
+        variates(S.seed, rk->key, vid, step, 0u, u1, u1b);
+        scatter = collides<R>(S, k, rk->conc, (R)s.vx, (R)s.vy, (R)s.vz, u1);
     }
     // 8. relocation and surface hit on the segment x -> x1.  Certified cases:
     //    x1 strictly inside the owner, or strictly outside exactly one face f
@@ -1255,7 +1263,10 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
             for (int ks = 0; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                StepResult r = step_one<R, kBatch, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
+                bool scatter = false;
+                StepResult r = step_one<R, kBatch, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink,
+                                                          scatter);
+                if (scatter) scatter_after<R>(S, k, rk, vid, step0 + (uint32_t)ks, s);
                 if (r.outcome == kFrozenNow) break;
                 ++processed;
                 moved = true;
@@ -1486,7 +1497,10 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
             for (int ks = it.k + 1; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                StepResult rr = step_one<R, true, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
+                bool scatter = false;
+                StepResult rr = step_one<R, true, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink,
+                                                         scatter);
+                if (scatter) scatter_after<R>(S, k, rk, vid, step0 + (uint32_t)ks, s);
                 if (rr.outcome == kFrozenNow) break;
                 ++extra;
                 if (S.trk_slot) track_start<R>(M, S, P.pidx[i], t_before, px, py, pz);
@@ -1716,6 +1730,24 @@ __global__ void orient_batch_kernel(const double* __restrict__ pts, int64_t n, i
     if (i >= n) return;
     const double* p = pts + 12 * i;
     out[i] = orient3d(p, p + 3, p + 6, p + 9);
+}
+
+// the shared elementary functions (dm_*) over n arguments, for checks:
+// which = 0 pow(x, y), 1 sin x and cos x, 2 acos x, 3 atan2(y, x), 4 log x, 5 exp x
+__global__ void math_batch_kernel(int which, const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                                  double* __restrict__ out, double* __restrict__ out2)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r = 0.0, r2 = 0.0;
+    if (which == 0) r = dm_pow(x[i], y[i]);
+    else if (which == 1) dm_sincos(x[i], &r, &r2);
+    else if (which == 2) r = dm_acos(x[i]);
+    else if (which == 3) r = dm_atan2(y[i], x[i]);
+    else if (which == 4) r = dm_log(x[i]);
+    else r = dm_exp(x[i]);
+    out[i] = r;
+    out2[i] = r2;
 }
 
 // absolute FP64 positions of the mixed layout, for read-back
@@ -2021,6 +2053,12 @@ void launch_e32(hipStream_t s, const double* E, float* E32, int64_t nt)
 void launch_orient_batch(hipStream_t s, const double* pts, int64_t n, int32_t* out)
 {
     if (n > 0) orient_batch_kernel<<<blocks_for(n, 64), 64, 0, s>>>(pts, n, out);
+}
+
+void launch_math_batch(hipStream_t s, int which, const double* x, const double* y, int64_t n, double* out,
+                       double* out2)
+{
+    if (n > 0) math_batch_kernel<<<blocks_for(n, 256), 256, 0, s>>>(which, x, y, n, out, out2);
 }
 
 void launch_abs_positions(hipStream_t s, const DevMesh& M, const float* lx, const float* ly, const float* lz,

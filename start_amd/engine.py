@@ -416,3 +416,19 @@ def orient3d_batch(pts: np.ndarray) -> np.ndarray:
     out = np.empty(len(pts), dtype=np.int32)
     _lib.check(_lib.lib().start_orient3d_batch(len(pts), _dp(pts), _ip(out)), None)
     return out
+
+
+MATH_FN = {"pow": 0, "sincos": 1, "acos": 2, "atan2": 3, "log": 4, "exp": 5}
+
+
+def math_batch(fn: str, x, y=None):
+    """The device's elementary functions of the collision path and the
+    particle sources over arrays: pow(x, y), sincos(x) -> (sin, cos),
+    acos(x), atan2(y, x), log(x), exp(x)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    y = np.ascontiguousarray(np.zeros_like(x) if y is None else y, dtype=np.float64).ravel()
+    if x.shape != y.shape:
+        raise ValueError("x and y must have the same length")
+    out, out2 = np.empty_like(x), np.empty_like(x)
+    _lib.check(_lib.lib().start_math_batch(MATH_FN[fn], len(x), _dp(x), _dp(y), _dp(out), _dp(out2)), None)
+    return (out, out2) if fn == "sincos" else out

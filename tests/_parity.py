@@ -12,9 +12,10 @@ from start_amd.species import E_CHARGE
 
 # Tolerances (FP64 path):
 #   owning tet, status, hit triangle, triangle counts: bit-exact
-#   x and v: |a - b| <= 1e-12 * |vector| per particle
+#   x and v: bit-exact (the collision scattering's pow, sin, cos, acos and
+#     atan2 are the same code on both sides, detmath.h)
 #   node charge: |a - b| <= 1e-12 * max(|Q_n|, e)   (atomics add in any order)
-REL_XV = 1e-12
+REL_XV = 0.0
 REL_Q = 1e-12
 
 # Tolerances (mixed path: FP32 positions and tables, FP64 velocities) against

@@ -1,7 +1,7 @@
 """Particles appended to a live population (start_append_particles; the
 continuous-sputtering spawn, ModelingMainDriver.cpp:141-142) against the oracle
 running the same appends: owners, settled flags, hit triangles and triangle
-counts bit for bit, x and v within 1e-12, node charge within 1e-12."""
+counts, x and v bit for bit, node charge within 1e-12."""
 from __future__ import annotations
 
 import numpy as np

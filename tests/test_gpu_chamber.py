@@ -64,8 +64,8 @@ def refresh(pr, w, step):
 @pytest.mark.parametrize("K", [1, 10])
 def test_chamber_fp64_parity_with_refresh(K):
     """FP64, charge every step, potential refreshed every 10 steps between
-    launches: owners, hits, statuses and triangle counts bit-exact, x and v
-    within 1e-12, node charge within 1e-12."""
+    launches: owners, hits, statuses, triangle counts, x and v bit-exact,
+    node charge within 1e-12."""
     w = small_chamber()
     pr = Pair(w, E=np.zeros((w.mesh.n_tets, 3)), steps_per_launch=K)
     for s in range(0, 60, w.refresh):

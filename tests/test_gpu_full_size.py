@@ -76,7 +76,7 @@ def test_full_size_sweep_fp64_parity():
     """The batched sweep at full size (64 runs of log-uniform size, 24.6M
     particles, 0-1000 V, 0.1-5 Pa VSS, on the 750,000-tet box), 10 steps in
     launches of 5 captured in graphs: every run bit-exact against its own
-    oracle run (owners, statuses, hits, per-run triangle counts; x, v and
+    oracle run (owners, statuses, hits, per-run triangle counts, x and v;
     per-run node charge within 1e-12)."""
     from oracle import pyoracle as O
     from ._parity import REL_Q, REL_XV

@@ -51,7 +51,7 @@ def mesh_workload(mesh, name, n=4000, seed=17):
 
 @pytest.mark.parametrize("name", ["cube", "two_cubes", "sphere", "plates"])
 def test_fp64_parity_on_gmsh_meshes(name):
-    """Owners, statuses, hits and triangle counts bit-exact, x, v and node
+    """Owners, statuses, hits, triangle counts, x and v bit-exact, node
     charge within 1e-12, over 3 x 20 fused steps."""
     w = gmsh_workload(name)
     pr = Pair(w, steps_per_launch=10)

@@ -84,7 +84,7 @@ def test_movements_match_the_oracle(precision, K, tmp_path):
     assert n_trunc > 100 and n_hits > 300
     if precision == "fp64":
         assert len(same) == len(expect)
-        tol_x, tol_h = 1e-12, 1e-9
+        tol_x, tol_h = 0.0, 1e-9
     else:
         assert len(same) >= 0.98 * len(expect)
         tol_x, tol_h = 1e-5, 1e-5

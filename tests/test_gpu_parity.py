@@ -1,7 +1,7 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle on the
 same seeded inputs and variates.  FP64 bar: owning tets, statuses, hit
-triangles and triangle counts bit-exact; x, v within 1e-12 of |x|, |v|; node
-charge within 1e-12 of max(|Q_n|, e)."""
+triangles, triangle counts, x and v bit-exact; node charge (summed by
+atomics in any order) within 1e-12 of max(|Q_n|, e)."""
 from __future__ import annotations
 
 import numpy as np

@@ -701,16 +701,22 @@ __device__ inline bool collides(const StepParams& S, const SpeciesK& k, double c
     return collides_exact64(S.model, k.sigma_hs, k.vhs_c, k.two_kt, k.m_r, k.vhs_exp, conc, S.dt, (double)vm, u1);
 }
 
-// post-collision velocity (rare: out of line to keep the streaming kernel
-// lean; arguments and result by value so nothing of the caller escapes to
-// memory)
+// Post-collision velocity of a particle whose collision test passed (rare:
+// about once per 1e4 particle-steps), out of line to keep the streaming kernel
+// lean.  The variates u2, u3 are drawn here, not by the caller, so that few of
+// the caller's registers are live at the call; arguments and result by value
+// so nothing of the caller escapes to memory.  FP64 in both precisions:
+// evaluated in FP32, sin(chi) = sqrt((1 - cos)(1 + cos)) of a forward scatter
+// (cos near 1 from an FP64 variate rounded to FP32) loses most of its digits.
 struct V3 {
     double x, y, z;
 };
-__device__ __noinline__ V3 collide_velocity(int model, double mass_cp, double mass_ct, double inv_alpha, double u2,
-                                            double u3, double vx, double vy, double vz)
+__device__ __noinline__ V3 collide_velocity(int model, const SpeciesK* k, uint64_t seed, uint32_t key, uint32_t vid,
+                                            uint32_t step, double vx, double vy, double vz)
 {
-    scatter_velocity(model, mass_cp, mass_ct, inv_alpha, u2, u3, vx, vy, vz);
+    double u2, u3;
+    variates(seed, key, vid, step, 1u, u2, u3);
+    scatter_velocity(model, k->mass_cp, k->mass_ct, k->inv_alpha, u2, u3, vx, vy, vz);
     return V3{vx, vy, vz};
 }
 
@@ -919,30 +925,10 @@ struct StepResult {
     int32_t hit_tri;
 };
 
-// Post-collision velocity of a particle whose collision test passed in this
-// step, with the FP64 expressions in both precisions (rare: about once per 1e4
-// particle-steps).  Evaluated in FP32, sin(chi) = sqrt((1 - cos)(1 + cos)) of
-// a forward scatter (cos near 1 from an FP64 variate rounded to FP32) loses
-// most of its digits.
-template <typename R>
-__device__ __forceinline__ void scatter_after(const StepParams& S, const SpeciesK& k, const RunK* __restrict__ rk,
-                                              uint32_t vid, uint32_t step, PState<R>& s)
-{
-    double u2, u3;
-    variates(S.seed, rk->key, vid, step, 1u, u2, u3);
-    V3 nv = collide_velocity(S.model, k.mass_cp, k.mass_ct, k.inv_alpha, u2, u3, s.vx, s.vy, s.vz);
-    s.vx = nv.x;
-    s.vy = nv.y;
-    s.vz = nv.z;
-}
-
-// The post-collision velocity is applied by the caller once step_one has
-// returned (scatter_after): nothing in the step after the collision test reads
-// v, and there few values are live across the out-of-line call.
 template <typename R, bool kBatch, bool kWalk, typename Sink>
 __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParams& S, const SpeciesK& k,
                                                const RunK* __restrict__ rk, uint32_t vid, uint32_t step,
-                                               int remaining, PState<R>& s, Sink& sink, bool& scatter)
+                                               int remaining, PState<R>& s, Sink& sink)
 {
     const double tau = tau_of<R>(M);
     // 2. |v| < 10 m/s and 3. outside the expanded bbox: frozen this step and
@@ -1008,12 +994,16 @@ __device__ __forceinline__ StepResult step_one(const DevMesh& M, const StepParam
         }
         return StepResult{kDone, -1};
     }
-    // 7. gas collision test (ParticlePhysicsUpdater.cpp:20-24); the new
-    //    velocity is scatter_after's
+    // 7. gas collision (ParticlePhysicsUpdater.cpp:20-24)
     if (S.model != 0) {
         double u1, u1b;
         variates(S.seed, rk->key, vid, step, 0u, u1, u1b);
-        scatter = collides<R>(S, k, rk->conc, (R)s.vx, (R)s.vy, (R)s.vz, u1);
+        if (collides<R>(S, k, rk->conc, (R)s.vx, (R)s.vy, (R)s.vz, u1)) {
+            V3 nv = collide_velocity(S.model, &k, S.seed, rk->key, vid, step, s.vx, s.vy, s.vz);
+            s.vx = nv.x;
+            s.vy = nv.y;
+            s.vz = nv.z;
+        }
     }
     // 8. relocation and surface hit on the segment x -> x1.  Certified cases:
     //    x1 strictly inside the owner, or strictly outside exactly one face f
@@ -1263,10 +1253,7 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
             for (int ks = 0; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                bool scatter = false;
-                StepResult r = step_one<R, kBatch, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink,
-                                                          scatter);
-                if (scatter) scatter_after<R>(S, k, rk, vid, step0 + (uint32_t)ks, s);
+                StepResult r = step_one<R, kBatch, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (r.outcome == kFrozenNow) break;
                 ++processed;
                 moved = true;
@@ -1497,10 +1484,7 @@ __global__ void __launch_bounds__(64) step_resolve(DevMesh M, DevParticles<R> P,
             for (int ks = it.k + 1; ks < S.nsteps; ++ks) {
                 const R px = s.x, py = s.y, pz = s.z;
                 const int32_t t_before = s.t;
-                bool scatter = false;
-                StepResult rr = step_one<R, true, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink,
-                                                         scatter);
-                if (scatter) scatter_after<R>(S, k, rk, vid, step0 + (uint32_t)ks, s);
+                StepResult rr = step_one<R, true, kWalk>(M, S, k, rk, vid, step0 + (uint32_t)ks, S.nsteps - ks, s, sink);
                 if (rr.outcome == kFrozenNow) break;
                 ++extra;
                 if (S.trk_slot) track_start<R>(M, S, P.pidx[i], t_before, px, py, pz);

@@ -920,6 +920,18 @@ struct PState {
 // frame of the unchanged owner; the caller resolves them exactly.
 // The owning tet's data is re-read each step (L1/L2 hits while the particle
 // stays in it) to keep the register footprint of the streaming kernel small.
+// The same index, through an empty asm statement (no instruction): the
+// compiler cannot reuse the per-lane addresses of the state loads at the start
+// of a step for the stores at its end, which kept eight 64-bit addresses live
+// across the FP64 step and spilled them to scratch; the stores recompute
+// theirs from the index and the kernel-argument bases instead.  (In the mixed
+// kernels the allocation without it measured faster.)
+__device__ __forceinline__ int64_t fresh_index(int64_t i)
+{
+    asm volatile("" : "+v"(i));
+    return i;
+}
+
 struct StepResult {
     int outcome;
     int32_t hit_tri;
@@ -1280,14 +1292,15 @@ __device__ __forceinline__ void step_tile(const DevMesh& M, const DevParticles<R
                 }
             }
             if (moved) {
-                P.x[i] = s.x;
-                P.y[i] = s.y;
-                P.z[i] = s.z;
-                P.vx[i] = s.vx;
-                P.vy[i] = s.vy;
-                P.vz[i] = s.vz;
-                if (s.t != t_in) P.tet[i] = s.t;
-                if (s.f != f_in) P.flags[i] = s.f;
+                const int64_t j = std::is_same<R, double>::value ? fresh_index(i) : i;
+                P.x[j] = s.x;
+                P.y[j] = s.y;
+                P.z[j] = s.z;
+                P.vx[j] = s.vx;
+                P.vy[j] = s.vy;
+                P.vz[j] = s.vz;
+                if (s.t != t_in) P.tet[j] = s.t;
+                if (s.f != f_in) P.flags[j] = s.f;
             }
         }
     }

@@ -120,6 +120,31 @@ def test_reduced_large_box_with_charge(model):
     pr.close()
 
 
+def test_vss_collisions_at_axis_aligned_velocities():
+    """VSS scattering where the rotation's angles are special values: the
+    velocities start along the axes (some with signed zeros), the field is
+    along z, and the gas is dense enough that nearly every particle collides
+    in the first step.  Then p_x = p_y = 0 gives atan2(+-0, +-0) and
+    acos(+-1); p along x or y gives atan2(0, p_x) and atan2(p_y, 0).
+    Positions and velocities bit-exact after the first step and after 30."""
+    w = workloads.large_box(n_particles=6000, cells=8, model="VSS")
+    w.gas.concentration *= 1000.0
+    w.dt = 2e-8
+    v = 2500.0
+    dirs = np.array([[0.0, 0.0, -v], [-0.0, 0.0, v], [0.0, -0.0, -v], [v, 0.0, 0.0], [-v, -0.0, 0.0],
+                     [0.0, v, 0.0], [-0.0, -v, -0.0]])
+    w.vel = np.ascontiguousarray(dirs[np.arange(len(w.pos)) % len(dirs)])
+    pr = Pair(w)
+    pr.advance(1)
+    r = pr.compare("axis-aligned VSS step 1")
+    moved = np.abs(np.stack([pr.st.vx, pr.st.vy, pr.st.vz], 1) - w.vel).max(1) > 1.0
+    assert moved.mean() > 0.9, moved.mean()  # nearly all scattered in the first step
+    pr.advance(30)
+    r = pr.compare("axis-aligned VSS step 31")
+    assert r["owner_changed"] > 20  # the dense gas keeps most particles in their tet
+    pr.close()
+
+
 def test_degenerate_starts_on_vertices_edges_faces():
     """Particles that start exactly on mesh vertices, edge midpoints and face
     centres (the owner tie rule and the exact grid-search path)."""

@@ -297,11 +297,13 @@ DM_FN double dm_cos(double x)
     return c;
 }
 
-/* x^y for x >= 0 (and NaN) */
+/* x^y for x >= 0 (and NaN); NaN for x < 0, where the C library returns a
+ * value only for integral y (the step never passes a negative base) */
 DM_FN double dm_pow(double x, double y)
 {
     if (y == 0.0) return 1.0;
     if (x != x || y != y) return x + y;
+    if (x < 0.0) return __builtin_nan("");
     if (x == 0.0) return y > 0.0 ? 0.0 : __builtin_inf();
     if (x == __builtin_inf()) return y > 0.0 ? x : 0.0;
     double lh, ll;

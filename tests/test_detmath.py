@@ -63,8 +63,8 @@ def test_pow_within_one_ulp():
         p, lp = both("pow", x, y)
         assert ulps(p, lp).max() <= 1.0
         assert (p == lp).mean() > 0.85
-    sx = np.array([0.0, 0.0, np.inf, np.inf, 1.0, 2.0, 0.25, 1e-300, np.nan, 3.0])
-    sy = np.array([0.3, -0.3, 0.3, -0.3, 0.7, 10.0, 0.5, 0.7, 0.5, 0.0])
+    sx = np.array([0.0, 0.0, np.inf, np.inf, 1.0, 2.0, 0.25, 1e-300, np.nan, 3.0, -1.5, -0.0])
+    sy = np.array([0.3, -0.3, 0.3, -0.3, 0.7, 10.0, 0.5, 0.7, 0.5, 0.0, 0.3, 0.7])
     p, lp = both("pow", sx, sy)
     assert np.array_equal(p, lp, equal_nan=True)
 

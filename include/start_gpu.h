@@ -356,6 +356,11 @@ int start_read_node_charge(start_ctx* ctx, double* out /* n_nodes */);
 #define START_DENSITY_TET_AVERAGE 1
 int start_node_charge_density(start_ctx* ctx, int32_t mode, double* rho /* n_nodes */,
                               int32_t* written /* n_nodes or NULL */);
+/* The same densities left in host buffers the context owns (page-locked):
+ * *rho (and *written unless written is NULL) point to n_nodes values, valid
+ * until the next density call on ctx or start_destroy.  Spares the caller a
+ * copy when it reads them once, as the C++ gather does when it fills its map. */
+int start_node_charge_density_view(start_ctx* ctx, int32_t mode, const double** rho, const int32_t** written);
 int start_read_particles(start_ctx* ctx, double* x, double* y, double* z, double* vx, double* vy,
                          double* vz, int32_t* tet, int32_t* status, int32_t* hit_tri);
 int start_read_tri_counts(start_ctx* ctx, uint64_t* counts /* n_tris */);

@@ -24,7 +24,7 @@ EXPORTS = [
     "start_gather_charge", "start_read_node_charge", "start_read_particles",
     "start_read_tri_counts", "start_read_settled", "start_read_stats", "start_synchronize",
     "start_locate", "start_orient3d_batch", "start_math_batch", "start_timing_enable", "start_timing_read",
-    "start_node_charge_density", "start_set_gas", "start_set_time_step",
+    "start_node_charge_density", "start_node_charge_density_view", "start_set_gas", "start_set_time_step",
     "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
     "start_read_node_charge_runs", "start_read_tri_counts_runs", "start_set_launch", "start_get_launch",
     "start_autotune", "start_autotune_cached", "start_record_settle", "start_read_hits", "start_adjoint_push",
@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
     L.start_timing_read.argtypes = [vp, P(d), P(u64)]
     L.start_timing_read_each.argtypes = [vp, P(d), P(i64)]
     L.start_node_charge_density.argtypes = [vp, i32, P(d), P(i32)]
+    L.start_node_charge_density_view.argtypes = [vp, i32, P(P(d)), P(P(i32))]
     L.start_set_gas.argtypes = [vp, P(GasDesc), P(d)]
     L.start_set_time_step.argtypes = [vp, d]
     L.start_load_batch.argtypes = [vp, i32, P(RunDesc)] + [P(d)] * 6 + [P(i32), P(u64)]

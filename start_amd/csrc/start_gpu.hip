@@ -2024,9 +2024,14 @@ int start_read_node_charge(start_ctx* c, double* out)
     return check_kernel_errors(c);
 }
 
-int start_node_charge_density(start_ctx* c, int32_t mode, double* rho, int32_t* written)
+}  // extern "C"
+
+namespace {
+
+// the node densities of the latest step into the context's pinned host
+// buffers h_rho (and h_written when asked), synchronously
+int node_density_to_host(start_ctx* c, int32_t mode, bool want_written)
 {
-    if (!c || !rho) return fail(c, START_ERR_ARG, "null argument");
     if (mode != START_DENSITY_BARYCENTRIC && mode != START_DENSITY_TET_AVERAGE)
         return fail(c, START_ERR_ARG, "unknown density mode");
     if (c->n_runs() != 1) return fail(c, START_ERR_STATE, "node charge density is defined for a single run");
@@ -2058,11 +2063,34 @@ int start_node_charge_density(start_ctx* c, int32_t mode, double* rho, int32_t* 
                                     c->d_written);
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipMemcpyAsync(c->h_rho, c->d_rho, sizeof(double) * M, hipMemcpyDeviceToHost, c->stream));
-    if (written)
+    if (want_written)
         HIPCHECK(c, hipMemcpyAsync(c->h_written, c->d_written, sizeof(int32_t) * M, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    std::memcpy(rho, c->h_rho, sizeof(double) * (size_t)M);
-    if (written) std::memcpy(written, c->h_written, sizeof(int32_t) * (size_t)M);
+    return START_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int start_node_charge_density(start_ctx* c, int32_t mode, double* rho, int32_t* written)
+{
+    if (!c || !rho) return fail(c, START_ERR_ARG, "null argument");
+    const int rc = node_density_to_host(c, mode, written != nullptr);
+    if (rc != START_OK) return rc;
+    const size_t M = (size_t)c->hm.n_nodes;
+    std::memcpy(rho, c->h_rho, sizeof(double) * M);
+    if (written) std::memcpy(written, c->h_written, sizeof(int32_t) * M);
+    return START_OK;
+}
+
+int start_node_charge_density_view(start_ctx* c, int32_t mode, const double** rho, const int32_t** written)
+{
+    if (!c || !rho) return fail(c, START_ERR_ARG, "null argument");
+    const int rc = node_density_to_host(c, mode, written != nullptr);
+    if (rc != START_OK) return rc;
+    *rho = c->h_rho;
+    if (written) *written = c->h_written;
     return START_OK;
 }
 

@@ -311,6 +311,16 @@ class ParticleInCellStep:
         self._check(_lib.lib().start_node_charge_density(self.ctx, _lib.DENSITY[mode], _dp(rho), _ip(wr)))
         return rho, wr.astype(bool)
 
+    def node_charge_density_view(self, mode: str = "barycentric"):
+        """The same through start_node_charge_density_view: copies of the
+        context's page-locked buffers (which the next density call reuses)."""
+        rp, wp = C.POINTER(C.c_double)(), C.POINTER(C.c_int32)()
+        self._check(_lib.lib().start_node_charge_density_view(self.ctx, _lib.DENSITY[mode], C.byref(rp),
+                                                               C.byref(wp)))
+        M = self.mesh.n_nodes
+        return (np.ctypeslib.as_array(rp, shape=(M,)).copy(),
+                np.ctypeslib.as_array(wp, shape=(M,)).astype(bool))
+
     def particles(self) -> dict:
         n = self.n
         f = {k: np.empty(n, dtype=np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}

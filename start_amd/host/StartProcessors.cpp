@@ -124,8 +124,6 @@ ParticleKernelDevice::ParticleKernelDevice(DeviceMeshInput mesh, DeviceKernelOpt
     for (size_t n = 0; n < M; ++n) m_rowsByTag[n] = n;
     std::sort(m_rowsByTag.begin(), m_rowsByTag.end(),
               [&](size_t a, size_t b) { return m_mesh.nodeTags[a] < m_mesh.nodeTags[b]; });
-    m_rho.resize(M);
-    m_written.resize(M);
 }
 
 void ParticleKernelDevice::useConfig(std::string_view path)
@@ -371,9 +369,12 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
         check(start_gather_charge(m_ctx), "charge scatter");
         m_charge_fresh = true;
     }
-    // barycentric: every node is written, so the written flags are not read
+    // barycentric: every node is written, so the written flags are not read.
+    // The values are read in place from the library's page-locked buffers.
     const bool all = m_opt.densityMode == START_DENSITY_BARYCENTRIC;
-    check(start_node_charge_density(m_ctx, m_opt.densityMode, m_rho.data(), all ? nullptr : m_written.data()),
+    const double* rho = nullptr;
+    const int32_t* written = nullptr;
+    check(start_node_charge_density_view(m_ctx, m_opt.densityMode, &rho, all ? nullptr : &written),
           "node charge density");
     // The driver keeps one map for the whole run and never clears it
     // (ModelingMainDriver.cpp:122), so after the first step every key is
@@ -392,14 +393,14 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
             auto it = out.lower_bound(m_mesh.nodeTags[m_rowsByTag[k0]]);
             for (size_t k = k0; k < k1 && !missing.load(std::memory_order_relaxed); ++k) {
                 const size_t r = m_rowsByTag[k];
-                if (!all && !m_written[r]) continue;
+                if (!all && !written[r]) continue;
                 const size_t tag = m_mesh.nodeTags[r];
                 while (it != out.end() && it->first < tag) ++it;
                 if (it == out.end() || it->first != tag) {
                     missing = true;
                     break;
                 }
-                it->second = m_rho[r];
+                it->second = rho[r];
                 ++it;
             }
         }
@@ -407,14 +408,14 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
     }
     auto it = out.begin();
     for (size_t r : m_rowsByTag) {
-        if (!all && !m_written[r]) continue;
+        if (!all && !written[r]) continue;
         const size_t tag = m_mesh.nodeTags[r];
         while (it != out.end() && it->first < tag) ++it;
         if (it != out.end() && it->first == tag) {
-            it->second = m_rho[r];
+            it->second = rho[r];
             ++it;
         } else {
-            it = std::next(out.emplace_hint(it, tag, m_rho[r]));
+            it = std::next(out.emplace_hint(it, tag, rho[r]));
         }
     }
 }

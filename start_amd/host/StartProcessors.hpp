@@ -183,8 +183,6 @@ private:
     DeviceMeshInput m_mesh;
     DeviceKernelOptions m_opt;
     std::vector<size_t> m_rowsByTag;     // node rows in ascending tag order
-    std::vector<double> m_rho;           // density read-back, by node row
-    std::vector<int32_t> m_written;
     start_ctx* m_ctx = nullptr;
     bool m_configured = false;
     bool m_config_read = false;

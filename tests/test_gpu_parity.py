@@ -75,6 +75,9 @@ def test_node_charge_density_modes():
     for prec in ("fp64", "mixed"):
         pr = Pair(w, precision=prec)
         pr.advance(40)
+        for mode in ("barycentric", "tet_average"):  # the in-place view gives the copied values
+            a, b = pr.eng.node_charge_density(mode), pr.eng.node_charge_density_view(mode)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), mode
         if prec == "fp64":
             pr.compare(f"density {prec} step {pr.step}")
             pr.compare_density(f"density {prec} step {pr.step}")

@@ -187,6 +187,9 @@ struct start_ctx {
     int32_t* d_written = nullptr;
     double* h_rho = nullptr;
     int32_t* h_written = nullptr;
+    unsigned long long* h_cnt = nullptr;  // page-locked copy of the counters (start_read_stats)
+    uint32_t* h_settle = nullptr;         // page-locked staging of settled load indices
+    int64_t h_settle_cap = 0;
     // particles: n live rows in buffers of cap rows (appends grow cap geometrically)
     int64_t n = 0;
     int64_t cap = 0;
@@ -1269,6 +1272,8 @@ void start_destroy(start_ctx* c)
     if (c->h_phi) (void)hipHostFree(c->h_phi);
     if (c->h_rho) (void)hipHostFree(c->h_rho);
     if (c->h_written) (void)hipHostFree(c->h_written);
+    if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+    if (c->h_settle) (void)hipHostFree(c->h_settle);
     if (c->phi_copied) (void)hipEventDestroy(c->phi_copied);
     for (auto& e : c->events) {
         (void)hipEventDestroy(e.a0);
@@ -2195,10 +2200,16 @@ int start_read_settled(start_ctx* c, uint64_t* ids, int64_t* n)
 int start_read_settled_since(start_ctx* c, int64_t first, uint64_t* ids, int64_t* n)
 {
     if (!c || !n || first < 0) return fail(c, START_ERR_ARG, "bad argument");
-    unsigned long long settled = 0;
-    HIPCHECK(c, hipMemcpyAsync(&settled, c->d_cnt + kCntSettled, sizeof(settled), hipMemcpyDeviceToHost, c->stream));
+    if (!c->h_cnt) {
+        hipError_t e = host_pinned_malloc((void**)&c->h_cnt, sizeof(unsigned long long) * kCntTotal);
+        if (e != hipSuccess)
+            return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                        std::string("statistics buffer: ") + hipGetErrorString(e));
+    }
+    HIPCHECK(c, hipMemcpyAsync(c->h_cnt + kCntSettled, c->d_cnt + kCntSettled, sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    const int64_t total = (int64_t)settled, m = std::max<int64_t>(0, total - first);
+    const int64_t total = (int64_t)c->h_cnt[kCntSettled], m = std::max<int64_t>(0, total - first);
     if (!ids) {
         *n = m;
         return START_OK;
@@ -2206,19 +2217,36 @@ int start_read_settled_since(start_ctx* c, int64_t first, uint64_t* ids, int64_t
     if (*n < m) return fail(c, START_ERR_ARG, "capacity below the number of newly settled particles");
     *n = m;
     if (m == 0) return START_OK;
-    std::vector<uint32_t> li((size_t)m);
-    HIPCHECK(c, hipMemcpyAsync(li.data(), c->d_settle_log + first, sizeof(uint32_t) * (size_t)m,
+    if (m > c->h_settle_cap) {
+        const int64_t cap = std::max<int64_t>(m, 4096);
+        uint32_t* p = nullptr;
+        hipError_t e = host_pinned_malloc((void**)&p, sizeof(uint32_t) * (size_t)cap);
+        if (e != hipSuccess)
+            return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                        std::string("settled-id staging: ") + hipGetErrorString(e));
+        if (c->h_settle) (void)hipHostFree(c->h_settle);
+        c->h_settle = p;
+        c->h_settle_cap = cap;
+    }
+    HIPCHECK(c, hipMemcpyAsync(c->h_settle, c->d_settle_log + first, sizeof(uint32_t) * (size_t)m,
                                hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    for (int64_t k = 0; k < m; ++k) ids[k] = c->ids[li[(size_t)k]];
+    for (int64_t k = 0; k < m; ++k) ids[k] = c->ids[c->h_settle[k]];
     return START_OK;
 }
 
 int start_read_stats(start_ctx* c, start_stats* out)
 {
     if (!c || !out) return fail(c, START_ERR_ARG, "null argument");
-    unsigned long long h[kCntTotal];
-    HIPCHECK(c, hipMemcpyAsync(h, c->d_cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    if (!c->h_cnt) {  // page-locked, so that the copy is one DMA behind the step
+        hipError_t e = host_pinned_malloc((void**)&c->h_cnt, sizeof(unsigned long long) * kCntTotal);
+        if (e != hipSuccess)
+            return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                        std::string("statistics buffer: ") + hipGetErrorString(e));
+    }
+    const unsigned long long* h = c->h_cnt;
+    HIPCHECK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(unsigned long long) * kCntTotal, hipMemcpyDeviceToHost,
+                               c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     memset(out, 0, sizeof(*out));
     out->step = c->step;

@@ -57,3 +57,17 @@ def test_device_math_equals_oracle_math(fn):
         assert same_bits(dev[0], ora[0]) and same_bits(dev[1], ora[1])
     else:
         assert same_bits(dev, ora)
+
+
+def test_math_batch_rejects_bad_arguments():
+    import ctypes as C
+
+    from start_amd import _lib
+    L = _lib.lib()
+    x = np.zeros(4)
+    p = x.ctypes.data_as(C.POINTER(C.c_double))
+    err_arg = -1  # START_ERR_ARG (include/start_gpu.h)
+    assert L.start_math_batch(6, 4, p, p, p, p) == err_arg
+    assert L.start_math_batch(-1, 4, p, p, p, p) == err_arg
+    assert L.start_math_batch(0, -1, p, p, p, p) == err_arg
+    assert L.start_math_batch(0, 0, None, None, None, None) == 0  # nothing to do

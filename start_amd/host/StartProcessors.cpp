@@ -376,9 +376,9 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
     const int32_t* written = nullptr;
     check(start_node_charge_density_view(m_ctx, m_opt.densityMode, &rho, all ? nullptr : &written),
           "node charge density");
-    // The driver keeps one map for the whole run and never clears it
-    // (ModelingMainDriver.cpp:122), so after the first step every key is
-    // already there and only its value changes.  Then the ordered pass runs in
+    // The reference's main loop keeps one map for the whole run and never
+    // clears it (ModelingMainDriver.cpp:122), so after the first step every
+    // key is already there and only its value changes.  Then the ordered pass runs in
     // chunks on several threads: each finds its first key, walks its part of
     // the map and writes values only (no insertion, so the tree is only read);
     // a written node without a key sends the whole pass to the serial loop

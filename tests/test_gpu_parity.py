@@ -78,6 +78,12 @@ def test_node_charge_density_modes():
         for mode in ("barycentric", "tet_average"):  # the in-place view gives the copied values
             a, b = pr.eng.node_charge_density(mode), pr.eng.node_charge_density_view(mode)
             assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), mode
+        import ctypes as C
+
+        from start_amd import _lib
+        rp = C.POINTER(C.c_double)()
+        assert _lib.lib().start_node_charge_density_view(pr.eng.ctx, 0, None, None) == -1  # START_ERR_ARG
+        assert _lib.lib().start_node_charge_density_view(pr.eng.ctx, 7, C.byref(rp), None) == -1  # unknown mode
         if prec == "fp64":
             pr.compare(f"density {prec} step {pr.step}")
             pr.compare_density(f"density {prec} step {pr.step}")

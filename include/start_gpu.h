@@ -373,6 +373,12 @@ int start_read_settled(start_ctx* ctx, uint64_t* ids, int64_t* n);
  * ids NULL: *n = the number of such particles; otherwise *n in = capacity. */
 int start_read_settled_since(start_ctx* ctx, int64_t first, uint64_t* ids, int64_t* n);
 int start_read_stats(start_ctx* ctx, start_stats* out);
+/* start_read_stats and start_read_settled_since(first) in one call, with one
+ * device sync when at most 1,024 particles settled since `first` (a second
+ * copy otherwise): what ParticleDynamicsProcessor::process reads after every
+ * step (ParticleSettler.cpp:7-26, the stop test at :24-25).  *n in = the
+ * capacity of ids, out = the number written. */
+int start_read_stats_settled(start_ctx* ctx, start_stats* out, int64_t first, uint64_t* ids, int64_t* n);
 int start_synchronize(start_ctx* ctx);
 
 /* Device point location of n points (owner rule of the step), for checks. */

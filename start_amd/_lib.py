@@ -22,7 +22,7 @@ EXPORTS = [
     "start_gas_concentration", "start_create", "start_destroy", "start_last_error",
     "start_load_particles", "start_set_tet_field", "start_set_node_potential", "start_step",
     "start_gather_charge", "start_read_node_charge", "start_read_particles",
-    "start_read_tri_counts", "start_read_settled", "start_read_stats", "start_synchronize",
+    "start_read_tri_counts", "start_read_settled", "start_read_stats", "start_read_stats_settled", "start_synchronize",
     "start_locate", "start_orient3d_batch", "start_math_batch", "start_timing_enable", "start_timing_read",
     "start_node_charge_density", "start_node_charge_density_view", "start_set_gas", "start_set_time_step",
     "start_load_batch", "start_set_tet_field_basis", "start_set_node_potential_basis",
@@ -118,6 +118,7 @@ def lib() -> C.CDLL:
     L.start_read_settled.argtypes = [vp, P(u64), P(i64)]
     L.start_read_settled_since.argtypes = [vp, i64, P(u64), P(i64)]
     L.start_read_stats.argtypes = [vp, P(Stats)]
+    L.start_read_stats_settled.argtypes = [vp, P(Stats), i64, P(u64), P(i64)]
     L.start_synchronize.argtypes = [vp]
     L.start_locate.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]
     L.start_locate_noalloc.argtypes = [vp, i64, P(d), P(d), P(d), P(i32)]

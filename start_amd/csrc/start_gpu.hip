@@ -2261,6 +2261,70 @@ int start_read_stats(start_ctx* c, start_stats* out)
     return START_OK;
 }
 
+int start_read_stats_settled(start_ctx* c, start_stats* out, int64_t first, uint64_t* ids, int64_t* n)
+{
+    if (!c || !out || !n || first < 0 || *n < 0 || (*n > 0 && !ids)) return fail(c, START_ERR_ARG, "bad argument");
+    // the counters and a window of the settle log after `first`, in one sync
+    const int64_t log_len = c->n;
+    const int64_t W = std::max<int64_t>(0, std::min<int64_t>({*n, (int64_t)1024, log_len - first}));
+    if (!c->h_cnt) {
+        hipError_t e = host_pinned_malloc((void**)&c->h_cnt, sizeof(unsigned long long) * kCntTotal);
+        if (e != hipSuccess)
+            return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                        std::string("statistics buffer: ") + hipGetErrorString(e));
+    }
+    if (std::max<int64_t>(W, 1) > c->h_settle_cap) {
+        const int64_t cap = std::max<int64_t>(W, 4096);
+        uint32_t* p = nullptr;
+        hipError_t e = host_pinned_malloc((void**)&p, sizeof(uint32_t) * (size_t)cap);
+        if (e != hipSuccess)
+            return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                        std::string("settled-id staging: ") + hipGetErrorString(e));
+        if (c->h_settle) (void)hipHostFree(c->h_settle);
+        c->h_settle = p;
+        c->h_settle_cap = cap;
+    }
+    HIPCHECK(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(unsigned long long) * kCntTotal, hipMemcpyDeviceToHost,
+                               c->stream));
+    if (W > 0)
+        HIPCHECK(c, hipMemcpyAsync(c->h_settle, c->d_settle_log + first, sizeof(uint32_t) * (size_t)W,
+                                   hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const unsigned long long* h = c->h_cnt;
+    memset(out, 0, sizeof(*out));
+    out->step = c->step;
+    for (int k = 0; k < kProcSlots; ++k) out->processed += h[kCntProcSlot0 + k];
+    out->settled = h[kCntSettled];
+    out->n_particles = (uint64_t)c->n;
+    out->deferred = h[kCntDeferred];
+    out->exhaustive = h[kCntExhaustive];
+    out->stop = (c->n > 0 && h[kCntSettled] >= (unsigned long long)c->n) ? 1 : 0;
+    if (h[kCntError] & 1ull) return fail(c, START_ERR_KERNEL, "exact predicate expansion capacity exceeded");
+    const int64_t m = std::max<int64_t>(0, (int64_t)h[kCntSettled] - first);
+    if (m > *n) return fail(c, START_ERR_ARG, "capacity below the number of newly settled particles");
+    *n = m;
+    if (m > W) {  // more than the window: the rest in a second copy
+        if (m > c->h_settle_cap) {
+            uint32_t* p = nullptr;
+            hipError_t e = host_pinned_malloc((void**)&p, sizeof(uint32_t) * (size_t)m);
+            if (e != hipSuccess)
+                return fail(c, e == hipErrorOutOfMemory ? START_ERR_NOMEM : START_ERR_HIP,
+                            std::string("settled-id staging: ") + hipGetErrorString(e));
+            if (c->h_settle) (void)hipHostFree(c->h_settle);
+            c->h_settle = p;
+            c->h_settle_cap = m;
+            HIPCHECK(c, hipMemcpyAsync(c->h_settle, c->d_settle_log + first, sizeof(uint32_t) * (size_t)m,
+                                       hipMemcpyDeviceToHost, c->stream));
+        } else {
+            HIPCHECK(c, hipMemcpyAsync(c->h_settle + W, c->d_settle_log + first + W, sizeof(uint32_t) * (size_t)(m - W),
+                                       hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    for (int64_t k = 0; k < m; ++k) ids[k] = c->ids[c->h_settle[k]];
+    return START_OK;
+}
+
 int start_synchronize(start_ctx* c)
 {
     if (!c) return fail(nullptr, START_ERR_ARG, "null context");

@@ -336,13 +336,14 @@ void ParticleKernelDevice::step(ParticleVector& particles, ParticlesIDSet& settl
     check(start_step(m_ctx, 1), "step");
     ++m_steps;
     m_charge_fresh = true;  // the step scatters the end-of-step charge
+    // the statistics and the ids settled since the last step, in one device sync
     start_stats s{};
-    check(start_read_stats(m_ctx, &s), "step statistics");
-    if (s.settled > m_settled_seen) {  // only the particles that settled since the last step
-        int64_t k = (int64_t)(s.settled - m_settled_seen);
-        std::vector<uint64_t> ids((size_t)k);
-        check(start_read_settled_since(m_ctx, (int64_t)m_settled_seen, ids.data(), &k), "settled ids");
-        settled.insert(ids.begin(), ids.begin() + k);
+    int64_t k = (int64_t)m_n - (int64_t)m_settled_seen;  // at most the particles not yet settled
+    if (m_settledIds.size() < (size_t)std::max<int64_t>(k, 1)) m_settledIds.resize((size_t)std::max<int64_t>(k, 1));
+    check(start_read_stats_settled(m_ctx, &s, (int64_t)m_settled_seen, m_settledIds.data(), &k),
+          "step statistics and settled ids");
+    if (k > 0) {
+        settled.insert(m_settledIds.begin(), m_settledIds.begin() + k);
         m_settled_seen = s.settled;
     }
     if (s.stop) stop.notifyStopRequested();  // ParticleSettler.cpp:24-25

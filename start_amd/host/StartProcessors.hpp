@@ -183,6 +183,7 @@ private:
     DeviceMeshInput m_mesh;
     DeviceKernelOptions m_opt;
     std::vector<size_t> m_rowsByTag;     // node rows in ascending tag order
+    std::vector<uint64_t> m_settledIds;  // ids settled in the last step (reused)
     start_ctx* m_ctx = nullptr;
     bool m_configured = false;
     bool m_config_read = false;

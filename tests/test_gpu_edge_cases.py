@@ -153,3 +153,38 @@ def test_autotune_records_no_movements():
     cnt = dict(zip(ids.tolist(), counts.tolist()))
     assert sorted(cnt) == sorted(expect) and all(cnt[i] == len(expect[i]) for i in expect)
     pr.close()
+
+
+def test_stats_and_settled_ids_in_one_call():
+    """start_read_stats_settled against start_read_stats + start_read_settled_since
+    when more particles settled than its one-sync window (1,024) holds, and more
+    than its first staging buffer (4,096): the same statistics and ids."""
+    import ctypes as C
+
+    w = workloads.small_box(n_particles=20000, charge_units=2.0)
+    eng = ParticleInCellStep(w.mesh, w.species, w.gas, w.dt, w.seed)
+    eng.load_particles(w.pos, w.vel, w.sp_idx)
+    eng.set_node_potential(w.phi)
+    eng.process(300)
+    L = _lib.lib()
+    ref = eng.stats()
+    total = int(ref["settled"])
+    log(f"stats and settled ids: {total} settled")
+    assert total > 5000
+    for first in (0, total - 2000, total - 700, total):
+        ids_a = np.zeros(total + 1, dtype=np.uint64)
+        na = C.c_int64(len(ids_a))
+        st = _lib.Stats()
+        assert L.start_read_stats_settled(eng.ctx, C.byref(st), first, ids_a.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          C.byref(na)) == 0, first
+        ids_b = np.zeros(total + 1, dtype=np.uint64)
+        nb = C.c_int64(len(ids_b))
+        assert L.start_read_settled_since(eng.ctx, first, ids_b.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          C.byref(nb)) == 0
+        assert na.value == nb.value == total - first
+        assert np.array_equal(ids_a[:na.value], ids_b[:nb.value]), first
+        assert st.settled == total and st.processed == ref["processed"] and st.stop == int(total == len(w.pos))
+    small = C.c_int64(10)  # a capacity below the count is an error, as in start_read_settled_since
+    assert L.start_read_stats_settled(eng.ctx, C.byref(_lib.Stats()), 0, np.zeros(10, dtype=np.uint64).ctypes.data_as(
+        C.POINTER(C.c_uint64)), C.byref(small)) == -1
+    eng.close()

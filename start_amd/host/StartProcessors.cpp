@@ -8,8 +8,24 @@
 #include <iostream>
 #include <iterator>
 #include <utility>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace start_amd {
+
+namespace {
+// threads for the ordered map passes: the OpenMP default (the host's cores or
+// OMP_NUM_THREADS), at most 16
+int mapThreads()
+{
+#ifdef _OPENMP
+    return std::max(1, std::min(16, omp_get_max_threads()));
+#else
+    return 1;
+#endif
+}
+}  // namespace
 
 namespace {
 
@@ -279,7 +295,7 @@ void ParticleKernelDevice::appendMovements(ParticleMovementMap& map, std::mutex&
     // its own particles' vectors.  Any missing key sends the step to the
     // serial loop below, which inserts.
     if (n >= 16384 && !map.empty()) {
-        const int T = 8;
+        const int T = mapThreads();
         std::vector<ParticleMovementMap::iterator> where((size_t)n, map.end());
         std::atomic<bool> missing{false};
 #pragma omp parallel for num_threads(T) schedule(static)
@@ -386,7 +402,7 @@ void ParticleKernelDevice::densities(NodeChargeDensitiesMap& out)
     // below, which inserts.  Keys of other nodes are kept.
     const size_t M = m_rowsByTag.size();
     if (out.size() >= M / 2 && M >= 16384) {
-        const int T = 8;
+        const int T = mapThreads();
         std::atomic<bool> missing{false};
 #pragma omp parallel for num_threads(T) schedule(static)
         for (int c = 0; c < T; ++c) {

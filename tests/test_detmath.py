@@ -147,3 +147,18 @@ def test_oracle_with_libm_and_shared_math_agree(model):
     assert not np.array_equal(va, vb)  # the two libraries do differ in the last bits
     assert np.abs(qa - qb).max() <= 1e-12 * np.abs(qb).max()
     assert (a.status == 1).sum() > 200
+
+
+def test_oracle_defaults_to_the_shared_functions():
+    """The GPU parity tests compare against the oracle's default mode, the
+    shared functions; or_set_libm switches to the C library and back."""
+    assert O.get_libm() is False
+    x = np.array([0.3, 1.7, 2.9])
+    shared = O.math_eval("sincos", x)[0]
+    O.set_libm(True)
+    try:
+        assert O.get_libm() is True
+    finally:
+        O.set_libm(False)
+    assert O.get_libm() is False
+    assert np.array_equal(O.math_eval("sincos", x)[0], shared)

@@ -81,7 +81,8 @@ def _device_field(w):
     return E
 
 
-def _oracle_loop(w, steps, charge_units, diameter, n, spawn_every=0, spawn=None, spawn_count=0):
+def _oracle_loop(w, steps, charge_units, diameter, n, spawn_every=0, spawn=None, spawn_count=0, model="HS",
+                 pressure=0.5):
     """The oracle stepped as shim_check drives the C++ loop: particles spawned
     at the start of every spawn_every-th step after the first, then one step.
     Returns the state, the simulation, the last step's node charge and the
@@ -94,10 +95,10 @@ def _oracle_loop(w, steps, charge_units, diameter, n, spawn_every=0, spawn=None,
     for t in TYPES:  # one species per particle type, as the C++ side sets them up
         q = charge_units * S.E_CHARGE if t == "Ti" else S.CHARGE_UNITS.get(t, 0) * S.E_CHARGE
         d = diameter if t == "Ti" else S.RADIUS[t] + S.RADIUS["Ar"]
-        ks.append(O.species_constants(S.MASS[t], q, d, gas_mass, S.VTI["Ar"], S.VSS_ALPHA["Ar"], O.MODEL["HS"]))
+        ks.append(O.species_constants(S.MASS[t], q, d, gas_mass, S.VTI["Ar"], S.VSS_ALPHA["Ar"], O.MODEL[model]))
     st = O.State.new(w.pos[:n], w.vel[:n], np.full(n, TYPES.index("Ti")), np.arange(n, dtype=np.uint64),
                      om.locate_many(w.pos[:n, 0], w.pos[:n, 1], w.pos[:n, 2]))
-    sim = O.Sim(om, ks, O.MODEL["HS"], S.concentration(0.5, 300.0), w.dt, w.seed, E=E)
+    sim = O.Sim(om, ks, O.MODEL[model], S.concentration(pressure, 300.0), w.dt, w.seed, E=E)
     q = np.zeros(w.mesh.n_nodes)
     expect = {}  # particleMovementMap: a particle joins when it first moves
     spawned = 0
@@ -167,6 +168,21 @@ def test_cpp_step_loop_matches_oracle(tmp_path, density):
     om, st, sim, q, expect = _oracle_loop(w, steps, charge_units, diameter, len(w.pos))
     ex, ev, eq, ns = _check(out, w, om, st, sim, q, expect, density, steps)
     log(f"shim {density}: ok, settled {ns}, max rel x {ex:.2e} v {ev:.2e} rho {eq:.2e}")
+
+
+@pytest.mark.parametrize("model", ["VHS", "VSS"])
+def test_cpp_step_loop_other_models_match_oracle(tmp_path, model):
+    """The C++ loop with the VHS and VSS models (named in lower case, as a run
+    config may write them) at 40x the gas density, so that collisions, and with
+    them the shared pow, sin, cos, acos and atan2, are frequent: bit-exact."""
+    charge_units, diameter, steps, pressure = 2.0, 2.0e-10, 60, 20.0
+    w = workloads.small_box(n_particles=2500, charge_units=charge_units)
+    out = _run_shim(tmp_path / model, w, steps, "barycentric", charge_units, diameter,
+                    extra={"model": model.lower(), "pressure": pressure})
+    om, st, sim, q, expect = _oracle_loop(w, steps, charge_units, diameter, len(w.pos), model=model,
+                                          pressure=pressure)
+    ex, ev, eq, ns = _check(out, w, om, st, sim, q, expect, "barycentric", steps)
+    log(f"shim {model}: ok, settled {ns}, max rel x {ex:.2e} v {ev:.2e} rho {eq:.2e}")
 
 
 @pytest.mark.parametrize("density", ["barycentric", "tet_average"])

@@ -237,6 +237,35 @@ def test_cpp_step_loop_takes_device_keys_from_the_run_config(tmp_path):
     log(f"shim run config: tet-average map from the JSON keys, settled {ns}, rho {eq:.2e}")
 
 
+def test_cpp_step_loop_in_mixed_precision(tmp_path):
+    """The C++ loop in mixed precision ("Device Precision": "Mixed" in the run
+    config) against the FP64 oracle, to the mixed bars of every mixed GPU
+    test: settled ids differ for at most 1e-3 of the particles, positions of
+    the particles whose status agrees within MIXED_REL of |x|, triangle counts
+    within the differing settles, and the density map's L1 within MIXED_Q_L1."""
+    from ._parity import MIXED_DIFFER, MIXED_Q_L1, MIXED_REL
+
+    charge_units, diameter, steps = 2.0, 2.0e-10, 60
+    w = workloads.small_box(n_particles=2500, charge_units=charge_units)
+    run_config = {"Mesh File": "box.msh", "Time Step": w.dt, "Gas": "Ar", "Model": "HS",
+                  "Device Precision": "Mixed", "Device Seed": w.seed}
+    out = _run_shim(tmp_path / "mixed", w, steps, "barycentric", charge_units, diameter, run_config=run_config)
+    om, st, sim, q, expect = _oracle_loop(w, steps, charge_units, diameter, len(w.pos))
+    n = len(w.pos)
+    settled_o = np.nonzero(st.status == O.SETTLED)[0] + 1000
+    differ = len(np.setxor1d(settled_o, out["settled"].astype(np.int64)))
+    assert differ <= max(1, MIXED_DIFFER * n), differ
+    same = np.isin(np.arange(1000, 1000 + n), out["settled"].astype(np.int64)) == (st.status == O.SETTLED)
+    xo = np.stack([st.x, st.y, st.z], 1)[same]
+    ex = (np.abs(out["pos"][same] - xo).max(1) / np.linalg.norm(xo, axis=1)).max()
+    assert ex <= MIXED_REL, ex
+    assert np.abs(out["tri"].astype(np.int64) - sim.tri_count[: w.mesh.n_tris].astype(np.int64)).sum() <= differ
+    ref = q / om.node_volumes()
+    l1 = np.abs(out["rho"] - ref).sum() / np.abs(ref).sum()
+    assert l1 <= MIXED_Q_L1, l1
+    log(f"shim mixed: settled {len(settled_o)}, differ {differ}, max rel x {ex:.2e}, density L1 {l1:.2e}")
+
+
 def test_cpp_step_loop_rejects_a_shrinking_vector(tmp_path):
     """Removing a particle from the vector is not something the reference
     loop does; the device keeps every particle, so it is reported, not
